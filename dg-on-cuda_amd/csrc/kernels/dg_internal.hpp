@@ -1,0 +1,72 @@
+// Library-internal handle types and launch helpers shared by the HIP
+// translation units behind the C ABI (include/dgc_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "dg_common.hpp"
+
+struct dgc_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+// Device copy of one discretization (mirror of d_simple_discretization_2d).
+struct dgc_disc_s {
+  dgc_ctx_s* ctx = nullptr;
+  int K = 0;
+  int order = 0;
+  int Np = 0, Nfp = 0;
+  int num_boundary_nodes = 0;
+  double* geo = nullptr;     // [kGeoComps][K]
+  int* nbr = nullptr;        // [3][K] packed connectivity
+  double* J = nullptr;       // [K]
+  double* face_J = nullptr;  // [3][K]
+  double* bx = nullptr;      // [num_boundary_nodes]
+  double* by = nullptr;
+  // host copies kept for set-up work that needs them (e.g. penalty factors)
+  std::vector<double> h_J, h_face_J;
+  std::vector<int> h_nbr;
+};
+
+namespace dgc {
+
+enum Physics { kMaxwellTM = 0, kAdvection = 1, kPhysicsCount = 2 };
+enum AdvectionBC { kBcSinSin = 0, kBcZero = 1 };
+
+// One fused RK4 stage (or a plain RHS evaluation when stage == 0).
+struct StageParams {
+  int K;
+  int bc;
+  const double* __restrict__ DrT;   // [Np][Npad] transposed, rows padded
+  const double* __restrict__ DsT;   // [Np][Npad]
+  const double* __restrict__ LT;    // [3Nfp][Npad]
+  const double* __restrict__ geo;
+  const int* __restrict__ nbr;
+  const double* __restrict__ bx;
+  const double* __restrict__ by;
+  const double* __restrict__ tclock;  // device time of the step start (nullptr: use t)
+  double ax, ay;
+  double t;       // stage time when tclock is null
+  double tfrac;   // stage time = tclock[0] + tfrac * dt_clock
+  double dt_clock;
+  double c1;      // stage coefficient (dt/2, dt/2, dt, dt/6)
+  const double* in[3];
+  double* out[3];
+  double* u[3];
+  double* acc[3];
+  double* k3[3];
+};
+
+// Padded operator row count used by the stage kernels.
+inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }
+
+hipError_t launch_stage(int order, int physics, int stage, const StageParams& p, hipStream_t s);
+
+}  // namespace dgc
+
+// Error reporting shared by every entry point.
+int dgc_set_error(int code, const std::string& msg);

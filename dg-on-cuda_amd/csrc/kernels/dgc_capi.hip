@@ -1,0 +1,556 @@
+// C ABI over the gfx950 kernels: contexts, device memory, discretizations,
+// semi-discrete operators and the RK4 time loop. See include/dgc_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/dgc_hip.h"
+#include "dg_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int dgc_set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define DGC_HIP_TRY(call)                                                                   \
+  do {                                                                                      \
+    hipError_t err_ = (call);                                                               \
+    if (err_ != hipSuccess)                                                                 \
+      return dgc_set_error(DGC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+#define DGC_REQUIRE(cond, msg) \
+  do {                         \
+    if (!(cond)) return dgc_set_error(DGC_ERR_INVALID, msg); \
+  } while (0)
+
+struct dgc_op_s {
+  dgc_disc_s* disc = nullptr;
+  int physics = 0;
+  int F = 0;
+  double* DrT = nullptr;
+  double* DsT = nullptr;
+  double* LT = nullptr;
+  double ax = 0.0, ay = 0.0;
+  int bc = 0;
+  double* tclock = nullptr;
+  // cached graph of one full RK4 step
+  hipGraphExec_t graph = nullptr;
+  std::vector<const void*> graph_key;
+  double graph_dt = 0.0;
+};
+
+extern "C" {
+
+const char* dgc_last_error(void) { return g_last_error.c_str(); }
+
+int dgc_max_order(void) { return dgc::kMaxOrder; }
+
+int dgc_ctx_create(int device, dgc_ctx_t* out) {
+  DGC_REQUIRE(out != nullptr, "dgc_ctx_create: out is null");
+  int n = 0;
+  DGC_HIP_TRY(hipGetDeviceCount(&n));
+  DGC_REQUIRE(device >= 0 && device < n, "dgc_ctx_create: no such device");
+  DGC_HIP_TRY(hipSetDevice(device));
+  auto* ctx = new dgc_ctx_s;
+  ctx->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return dgc_set_error(DGC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return DGC_OK;
+}
+
+int dgc_ctx_destroy(dgc_ctx_t ctx) {
+  if (ctx == nullptr) return DGC_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DGC_OK;
+}
+
+int dgc_ctx_sync(dgc_ctx_t ctx) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_ctx_sync: null context");
+  DGC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_malloc(dgc_ctx_t ctx, size_t bytes, void** ptr) {
+  DGC_REQUIRE(ctx != nullptr && ptr != nullptr, "dgc_malloc: null argument");
+  DGC_HIP_TRY(hipSetDevice(ctx->device));
+  DGC_HIP_TRY(hipMalloc(ptr, bytes == 0 ? 1 : bytes));
+  return DGC_OK;
+}
+
+int dgc_free(dgc_ctx_t ctx, void* ptr) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_free: null context");
+  if (ptr == nullptr) return DGC_OK;
+  DGC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  DGC_HIP_TRY(hipFree(ptr));
+  return DGC_OK;
+}
+
+int dgc_memcpy_h2d(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_memcpy_h2d: null context");
+  DGC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  DGC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_memcpy_d2h(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_memcpy_d2h: null context");
+  DGC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  DGC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_memcpy_d2d(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_memcpy_d2d: null context");
+  DGC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_memset_zero(dgc_ctx_t ctx, void* dst, size_t bytes) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_memset_zero: null context");
+  DGC_HIP_TRY(hipMemsetAsync(dst, 0, bytes, ctx->stream));
+  return DGC_OK;
+}
+
+// ---------------------------------------------------------------------------
+int dgc_disc2d_create(dgc_ctx_t ctx, int num_cells, int order, const int* face0_nodes,
+                      const int* face1_nodes, const int* face2_nodes, const double* inv_jac,
+                      const double* J, const double* face_J, const int* iface_cell,
+                      const int* iface_face, int num_boundary_nodes, const double* bx,
+                      const double* by, const double* nx, const double* ny, dgc_disc_t* out) {
+  DGC_REQUIRE(ctx != nullptr && out != nullptr, "dgc_disc2d_create: null argument");
+  DGC_REQUIRE(num_cells > 0 && num_cells < (1 << 29), "dgc_disc2d_create: num_cells out of range");
+  if (order < 1 || order > dgc::kMaxOrder)
+    return dgc_set_error(DGC_ERR_UNSUPPORTED, "dgc_disc2d_create: order must be in [1, " +
+                                                  std::to_string(dgc::kMaxOrder) + "]");
+  DGC_REQUIRE(inv_jac && J && face_J && iface_cell && iface_face && nx && ny,
+              "dgc_disc2d_create: null geometry array");
+  DGC_REQUIRE(num_boundary_nodes >= 0, "dgc_disc2d_create: negative boundary node count");
+  DGC_REQUIRE(num_boundary_nodes == 0 || (bx && by), "dgc_disc2d_create: null boundary nodes");
+  const int K = num_cells, Nfp = order + 1;
+  // The face node tables are fixed by the node ordering; verify the caller's.
+  const int* fn[3] = {face0_nodes, face1_nodes, face2_nodes};
+  for (int f = 0; f < 3; ++f) {
+    if (fn[f] == nullptr) continue;
+    for (int d = 0; d < Nfp; ++d) {
+      const int Np = (order + 1) * (order + 2) / 2;
+      const int want = f == 0 ? d : (f == 1 ? order + d * order - d * (d - 1) / 2 : Np - 1 - d * (d + 3) / 2);
+      DGC_REQUIRE(fn[f][d] == want, "dgc_disc2d_create: face node table does not match the node ordering");
+    }
+  }
+  std::vector<double> geo(size_t(dgc::kGeoComps) * K);
+  std::vector<int> nbr(size_t(3) * K);
+  for (int c = 0; c < K; ++c) {
+    if (!(J[c] > 0.0) || !std::isfinite(J[c]))
+      return dgc_set_error(DGC_ERR_INVALID, "dgc_disc2d_create: cell " + std::to_string(c) +
+                                                " is clockwise or degenerate (J <= 0)");
+    geo[size_t(dgc::kRx) * K + c] = inv_jac[4 * size_t(c) + 0];
+    geo[size_t(dgc::kRy) * K + c] = inv_jac[4 * size_t(c) + 1];
+    geo[size_t(dgc::kSx) * K + c] = inv_jac[4 * size_t(c) + 2];
+    geo[size_t(dgc::kSy) * K + c] = inv_jac[4 * size_t(c) + 3];
+    for (int e = 0; e < 3; ++e) {
+      const size_t h = 3 * size_t(c) + e;
+      geo[size_t(dgc::kFs0 + e) * K + c] = face_J[h] / J[c];
+      geo[size_t(dgc::kNx0 + e) * K + c] = nx[h];
+      geo[size_t(dgc::kNy0 + e) * K + c] = ny[h];
+      const int nc = iface_cell[h], nf = iface_face[h];
+      if (nc == c) {
+        DGC_REQUIRE(nf >= 0 && nf + Nfp <= num_boundary_nodes,
+                    "dgc_disc2d_create: boundary offset out of range at cell " + std::to_string(c));
+        nbr[size_t(e) * K + c] = (nf << 2) | dgc::kBoundaryCode;
+      } else {
+        DGC_REQUIRE(nc >= 0 && nc < K && nf >= 0 && nf < 3,
+                    "dgc_disc2d_create: neighbour out of range at cell " + std::to_string(c));
+        nbr[size_t(e) * K + c] = (nc << 2) | nf;
+      }
+    }
+  }
+  DGC_HIP_TRY(hipSetDevice(ctx->device));
+  auto* d = new dgc_disc_s;
+  d->ctx = ctx;
+  d->K = K;
+  d->order = order;
+  d->Np = (order + 1) * (order + 2) / 2;
+  d->Nfp = Nfp;
+  d->num_boundary_nodes = num_boundary_nodes;
+  d->h_J.assign(J, J + K);
+  d->h_face_J.assign(face_J, face_J + 3 * size_t(K));
+  d->h_nbr = nbr;
+  auto fail = [&](hipError_t e, const char* what) {
+    hipFree(d->geo);
+    hipFree(d->nbr);
+    hipFree(d->J);
+    hipFree(d->face_J);
+    hipFree(d->bx);
+    hipFree(d->by);
+    delete d;
+    return dgc_set_error(DGC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&d->geo, geo.size() * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&d->nbr, nbr.size() * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&d->J, K * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&d->face_J, 3 * size_t(K) * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc");
+  const size_t nb = size_t(num_boundary_nodes > 0 ? num_boundary_nodes : 1);
+  if ((e = hipMalloc(&d->bx, nb * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&d->by, nb * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc");
+  std::vector<double> fJ(3 * size_t(K));
+  for (int c = 0; c < K; ++c)
+    for (int f = 0; f < 3; ++f) fJ[size_t(f) * K + c] = face_J[3 * size_t(c) + f];
+  if ((e = hipMemcpy(d->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(d->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(d->J, J, K * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(d->face_J, fJ.data(), fJ.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if (num_boundary_nodes > 0) {
+    if ((e = hipMemcpy(d->bx, bx, num_boundary_nodes * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "hipMemcpy");
+    if ((e = hipMemcpy(d->by, by, num_boundary_nodes * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "hipMemcpy");
+  }
+  *out = d;
+  return DGC_OK;
+}
+
+int dgc_disc2d_destroy(dgc_disc_t disc) {
+  if (disc == nullptr) return DGC_OK;
+  hipSetDevice(disc->ctx->device);
+  hipStreamSynchronize(disc->ctx->stream);
+  hipFree(disc->geo);
+  hipFree(disc->nbr);
+  hipFree(disc->J);
+  hipFree(disc->face_J);
+  hipFree(disc->bx);
+  hipFree(disc->by);
+  delete disc;
+  return DGC_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int upload_operators(dgc_op_s* op, const double* Dr, const double* Ds, const double* lift) {
+  dgc_disc_s* d = op->disc;
+  const int Np = d->Np, NF = 3 * d->Nfp, Npad = dgc::padded_rows(Np);
+  std::vector<double> drT(size_t(Np) * Npad, 0.0), dsT(size_t(Np) * Npad, 0.0), lT(size_t(NF) * Npad, 0.0);
+  for (int j = 0; j < Np; ++j) {
+    for (int k = 0; k < Np; ++k) {
+      drT[size_t(k) * Npad + j] = Dr[size_t(j) * Np + k];
+      dsT[size_t(k) * Npad + j] = Ds[size_t(j) * Np + k];
+    }
+    for (int m = 0; m < NF; ++m) lT[size_t(m) * Npad + j] = lift[size_t(j) * NF + m];
+  }
+  DGC_HIP_TRY(hipMalloc(&op->DrT, drT.size() * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&op->DsT, dsT.size() * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&op->LT, lT.size() * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&op->tclock, 2 * sizeof(double)));
+  DGC_HIP_TRY(hipMemcpy(op->DrT, drT.data(), drT.size() * sizeof(double), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(op->DsT, dsT.data(), dsT.size() * sizeof(double), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(op->LT, lT.data(), lT.size() * sizeof(double), hipMemcpyHostToDevice));
+  return DGC_OK;
+}
+
+static void free_op(dgc_op_s* op) {
+  if (op->graph) hipGraphExecDestroy(op->graph);
+  hipFree(op->DrT);
+  hipFree(op->DsT);
+  hipFree(op->LT);
+  hipFree(op->tclock);
+  delete op;
+}
+
+static int create_op(dgc_disc_t disc, int physics, const double* Dr, const double* Ds,
+                     const double* lift, double ax, double ay, int bc, dgc_op_t* out) {
+  DGC_REQUIRE(disc != nullptr && out != nullptr, "create operator: null argument");
+  DGC_REQUIRE(Dr && Ds && lift, "create operator: null operator matrix");
+  for (size_t i = 0; i < size_t(disc->Np) * disc->Np; ++i)
+    DGC_REQUIRE(std::isfinite(Dr[i]) && std::isfinite(Ds[i]), "create operator: non-finite Dr/Ds");
+  DGC_HIP_TRY(hipSetDevice(disc->ctx->device));
+  auto* op = new dgc_op_s;
+  op->disc = disc;
+  op->physics = physics;
+  op->F = physics == dgc::kMaxwellTM ? 3 : 1;
+  op->ax = ax;
+  op->ay = ay;
+  op->bc = bc;
+  int rc = upload_operators(op, Dr, Ds, lift);
+  if (rc != DGC_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return DGC_OK;
+}
+
+int dgc_maxwell_tm_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                          dgc_op_t* out) {
+  return create_op(disc, dgc::kMaxwellTM, Dr, Ds, lift, 0.0, 0.0, 0, out);
+}
+
+int dgc_advection_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                         double ax, double ay, int bc, dgc_op_t* out) {
+  DGC_REQUIRE(bc == dgc::kBcSinSin || bc == dgc::kBcZero, "dgc_advection_create: unknown bc");
+  DGC_REQUIRE(std::isfinite(ax) && std::isfinite(ay), "dgc_advection_create: non-finite velocity");
+  return create_op(disc, dgc::kAdvection, Dr, Ds, lift, ax, ay, bc, out);
+}
+
+int dgc_op_destroy(dgc_op_t op) {
+  if (op == nullptr) return DGC_OK;
+  hipSetDevice(op->disc->ctx->device);
+  hipStreamSynchronize(op->disc->ctx->stream);
+  free_op(op);
+  return DGC_OK;
+}
+
+int dgc_op_num_fields(dgc_op_t op) { return op ? op->F : 0; }
+
+static dgc::StageParams base_params(const dgc_op_s* op) {
+  dgc::StageParams p{};
+  const dgc_disc_s* d = op->disc;
+  p.K = d->K;
+  p.bc = op->bc;
+  p.DrT = op->DrT;
+  p.DsT = op->DsT;
+  p.LT = op->LT;
+  p.geo = d->geo;
+  p.nbr = d->nbr;
+  p.bx = d->bx;
+  p.by = d->by;
+  p.ax = op->ax;
+  p.ay = op->ay;
+  p.tclock = nullptr;
+  return p;
+}
+
+static int check_fields(const dgc_op_s* op, const double* const* v, const char* what) {
+  if (v == nullptr) return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": null field array");
+  for (int f = 0; f < op->F; ++f)
+    if (v[f] == nullptr) return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": null field pointer");
+  return DGC_OK;
+}
+
+int dgc_op_rhs(dgc_op_t op, const double* const* in, size_t size, double t, double* const* out) {
+  DGC_REQUIRE(op != nullptr, "dgc_op_rhs: null operator");
+  DGC_REQUIRE(size == size_t(op->disc->K) * op->disc->Np, "dgc_op_rhs: size != K * Np");
+  int rc;
+  if ((rc = check_fields(op, in, "dgc_op_rhs")) != DGC_OK) return rc;
+  if ((rc = check_fields(op, out, "dgc_op_rhs")) != DGC_OK) return rc;
+  dgc::StageParams p = base_params(op);
+  p.t = t;
+  for (int f = 0; f < op->F; ++f) {
+    p.in[f] = in[f];
+    p.out[f] = out[f];
+  }
+  hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 0, p, op->disc->ctx->stream);
+  if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
+  return DGC_OK;
+}
+
+__global__ void axpy_kernel(double a, const double* __restrict__ x, size_t n,
+                            const double* __restrict__ y, double* __restrict__ out) {
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = a * x[i] + y[i];
+}
+
+int dgc_axpy(dgc_ctx_t ctx, double a, const double* x, size_t n, const double* y, double* out) {
+  DGC_REQUIRE(ctx != nullptr, "dgc_axpy: null context");
+  if (n == 0) return DGC_OK;
+  DGC_REQUIRE(x && y && out, "dgc_axpy: null pointer");
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(axpy_kernel, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, a, x, n, y, out);
+  if (hipGetLastError() != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
+  return DGC_OK;
+}
+
+// One fused RK4 step. wk mapping: wk0 = stage accumulator (k1 + 2 k2),
+// wk1 / wk2 = stage inputs (ping-pong), wk3 = k3. wk4 is unused.
+static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, double* const* wk,
+                        bool use_clock) {
+  const int F = op->F;
+  dgc::StageParams p = base_params(op);
+  for (int f = 0; f < F; ++f) {
+    p.u[f] = u[f];
+    p.acc[f] = wk[0 * F + f];
+    p.k3[f] = wk[3 * F + f];
+  }
+  if (use_clock) {
+    p.tclock = op->tclock;
+    p.dt_clock = dt;
+  }
+  const double half = 0.5 * dt, sixth = dt / 6.0;
+  struct Stage {
+    int id;
+    double tfrac;
+    double c1;
+    int in_buf;   // -1: state, else work index
+    int out_buf;  // -1: state
+  } stages[4] = {{1, 0.0, half, -1, 1}, {2, 0.5, half, 1, 2}, {3, 0.5, dt, 2, 1}, {4, 1.0, sixth, 1, -1}};
+  for (const Stage& s : stages) {
+    p.t = (s.tfrac == 0.0) ? t : (s.tfrac == 1.0 ? t + dt : t + 0.5 * dt);
+    p.tfrac = s.tfrac;
+    p.c1 = s.c1;
+    for (int f = 0; f < F; ++f) {
+      p.in[f] = s.in_buf < 0 ? u[f] : wk[s.in_buf * F + f];
+      p.out[f] = s.out_buf < 0 ? u[f] : wk[s.out_buf * F + f];
+    }
+    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, p, op->disc->ctx->stream);
+    if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
+  }
+  return DGC_OK;
+}
+
+__global__ void tick_kernel(double* tclock, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) tclock[0] = tclock[0] + dt;
+}
+
+static int check_step_args(dgc_op_s* op, double* const* inout, size_t size, double* const* wk,
+                           const char* what) {
+  if (op == nullptr) return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": null operator");
+  if (size != size_t(op->disc->K) * op->disc->Np)
+    return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": size != K * Np");
+  int rc;
+  if ((rc = check_fields(op, inout, what)) != DGC_OK) return rc;
+  if (wk == nullptr) return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": null work array");
+  for (int i = 0; i < 4 * op->F; ++i)
+    if (wk[i] == nullptr) return dgc_set_error(DGC_ERR_INVALID, std::string(what) + ": null work vector");
+  return DGC_OK;
+}
+
+int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, double dt,
+                 double* const* wk) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_rk4_step");
+  if (rc != DGC_OK) return rc;
+  return enqueue_step(op, inout, t, dt, wk, false);
+}
+
+// Only the advection operator with the analytic inflow data reads the time.
+static bool op_uses_time(const dgc_op_s* op) {
+  return op->physics == dgc::kAdvection && op->bc == dgc::kBcSinSin && op->disc->num_boundary_nodes > 0;
+}
+
+// Build (or reuse) a graph holding one full step driven by the device clock.
+static int step_graph(dgc_op_s* op, double* const* u, double dt, double* const* wk) {
+  std::vector<const void*> key;
+  for (int f = 0; f < op->F; ++f) key.push_back(u[f]);
+  for (int i = 0; i < 4 * op->F; ++i) key.push_back(wk[i]);
+  if (op->graph && key == op->graph_key && op->graph_dt == dt) return DGC_OK;
+  if (op->graph) {
+    hipGraphExecDestroy(op->graph);
+    op->graph = nullptr;
+  }
+  hipStream_t s = op->disc->ctx->stream;
+  hipGraph_t g = nullptr;
+  DGC_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(op, u, 0.0, dt, wk, true);
+  if (rc == DGC_OK && op_uses_time(op)) {
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, s, op->tclock, dt);
+  }
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != DGC_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&op->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    op->graph = nullptr;
+    return dgc_set_error(DGC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  op->graph_key = key;
+  op->graph_dt = dt;
+  return DGC_OK;
+}
+
+static int run_full_steps(dgc_op_s* op, double* const* u, double t0, double dt, int nsteps,
+                          double* const* wk) {
+  if (nsteps <= 0) return DGC_OK;
+  int rc = step_graph(op, u, dt, wk);
+  if (rc != DGC_OK) return rc;
+  hipStream_t s = op->disc->ctx->stream;
+  DGC_HIP_TRY(hipMemcpyAsync(op->tclock, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  // the host copy must stay alive until the copy has run
+  DGC_HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < nsteps; ++i) DGC_HIP_TRY(hipGraphLaunch(op->graph, s));
+  return DGC_OK;
+}
+
+int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, double dt, int nsteps,
+                  double* const* wk) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_rk4_steps");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(nsteps >= 0, "dgc_rk4_steps: negative step count");
+  DGC_REQUIRE(std::isfinite(dt) && dt > 0.0, "dgc_rk4_steps: dt must be positive");
+  return run_full_steps(op, inout, t0, dt, nsteps, wk);
+}
+
+int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
+                        int nsteps, double* const* wk, float* ms) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_rk4_steps_timed");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(nsteps > 0 && ms != nullptr, "dgc_rk4_steps_timed: need nsteps > 0 and ms");
+  DGC_REQUIRE(std::isfinite(dt) && dt > 0.0, "dgc_rk4_steps_timed: dt must be positive");
+  rc = step_graph(op, inout, dt, wk);
+  if (rc != DGC_OK) return rc;
+  hipStream_t s = op->disc->ctx->stream;
+  DGC_HIP_TRY(hipMemcpyAsync(op->tclock, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+  DGC_HIP_TRY(hipStreamSynchronize(s));
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  hipError_t err = hipEventRecord(e0, s);
+  for (int i = 0; i < nsteps && err == hipSuccess; ++i) err = hipGraphLaunch(op->graph, s);
+  if (err == hipSuccess) err = hipEventRecord(e1, s);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  if (err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("timed replay: ") + hipGetErrorString(err));
+  return DGC_OK;
+}
+
+int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, double t_end, double dt,
+                double* const* wk, double* t_out, int* steps_out) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_rk4_run");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(std::isfinite(dt) && dt > 0.0, "dgc_rk4_run: dt must be positive");
+  DGC_REQUIRE(std::isfinite(t0) && std::isfinite(t_end), "dgc_rk4_run: non-finite time");
+  // Replay the host loop `while (t < T) { if (t + dt > T) dt = T - t; step; t += dt; }`
+  // to count full steps; the device clock accumulates the same sums.
+  double t = t0;
+  int nfull = 0;
+  while (t < t_end && !(t + dt > t_end)) {
+    t += dt;
+    ++nfull;
+  }
+  rc = run_full_steps(op, inout, t0, dt, nfull, wk);
+  if (rc != DGC_OK) return rc;
+  int steps = nfull;
+  if (t < t_end) {
+    const double last = t_end - t;
+    rc = enqueue_step(op, inout, t, last, wk, false);
+    if (rc != DGC_OK) return rc;
+    t += last;
+    ++steps;
+  }
+  if (t_out) *t_out = t;
+  if (steps_out) *steps_out = steps;
+  return DGC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+/*
+ * dgc_hip.h -- C ABI of the MI355X (gfx950) nodal-DG hot path.
+ *
+ * Plain pointers and sizes only. Every entry point returns an int status
+ * (DGC_OK == 0); dgc_last_error() gives the message of the last failure on
+ * the calling thread. Device work is queued on the context's stream and is
+ * asynchronous unless stated otherwise; handles are not thread-safe.
+ *
+ * Array conventions follow the reference:
+ *   - DOFs on the device are node-major per field: u[j * K + c]
+ *     (examples/maxwell_2d/maxwell_2d.h:129-135, the !USE_CPU_ONLY branch);
+ *   - host geometry arrays are the outputs of simple_discretization_2d::fill_*
+ *     (src/simple_discretization_2d.h:112-218): inv_jac[4K] = (rx, ry, sx, sy),
+ *     J[K], face_J[3K], iface_cell/iface_face[3K] (boundary face: cell itself and
+ *     the offset into the boundary-node arrays), nx/ny[3K];
+ *   - operator matrices are row-major (src/dense_matrix.h, CM = false).
+ */
+#ifndef DGC_HIP_H
+#define DGC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dgc_status {
+  DGC_OK = 0,
+  DGC_ERR_INVALID = 1,       /* bad argument (size, order, null pointer) */
+  DGC_ERR_HIP = 2,           /* a HIP runtime call failed */
+  DGC_ERR_LAUNCH = 3,        /* kernel launch failed */
+  DGC_ERR_NONFINITE = 4,     /* NaN/Inf detected in a checked result */
+  DGC_ERR_NOT_CONVERGED = 5, /* iterative solver hit its iteration cap */
+  DGC_ERR_UNSUPPORTED = 6    /* configuration not built into this library */
+};
+
+typedef struct dgc_ctx_s* dgc_ctx_t;
+typedef struct dgc_disc_s* dgc_disc_t;
+typedef struct dgc_op_s* dgc_op_t;
+typedef struct dgc_mesh_s* dgc_mesh_t;
+
+/* ---- errors / version -------------------------------------------------- */
+const char* dgc_last_error(void);
+int dgc_max_order(void);
+
+/* ---- context and device memory ------------------------------------------
+ * Replaces the legacy default stream and the thrust::device_vector /
+ * cudaMallocManaged buffers of examples/maxwell_2d/main.cpp:103-114,177-197. */
+int dgc_ctx_create(int device, dgc_ctx_t* out);
+int dgc_ctx_destroy(dgc_ctx_t ctx);
+int dgc_ctx_sync(dgc_ctx_t ctx);
+int dgc_malloc(dgc_ctx_t ctx, size_t bytes, void** ptr);
+int dgc_free(dgc_ctx_t ctx, void* ptr);
+int dgc_memcpy_h2d(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int dgc_memcpy_d2h(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int dgc_memcpy_d2d(dgc_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int dgc_memset_zero(dgc_ctx_t ctx, void* dst, size_t bytes);
+
+/* ---- host set-up (reference element, mesh, geometry) --------------------
+ * dgc_ref_operators: nodes, face node lists and the operator matrices of
+ * order `order` in weak form (form 0; examples/maxwell_2d/maxwell_2d.h:79-112,
+ * examples/advection_2d/advection_2d.h:74-101) or strong form (form 1;
+ * src/laplace_2d.h:87-112). Sizes: r, s [Np]; fmask [3 Nfp]; Dr, Ds, mass
+ * [Np*Np]; lift [Np*3Nfp]. Any output pointer may be NULL. */
+int dgc_ref_operators(int order, int form, double* r, double* s, int* fmask, double* Dr, double* Ds,
+                      double* lift, double* mass);
+
+/* Structured jittered triangle mesh (the generated meshes of the benchmark
+ * configurations); topology as src/mesh/simple_triangular_mesh_2d.h:151-217. */
+int dgc_mesh_structured(int nx, int ny, double x0, double x1, double y0, double y1, double jitter,
+                        uint64_t seed, int periodic, dgc_mesh_t* out);
+/* Mesh from vertex coordinates and triangle connectivity
+ * (src/mesh/simple_triangular_mesh_2d.h:60-80; vertices kept as (x, y)). */
+int dgc_mesh_from_arrays(int nv, const double* vx, const double* vy, int k, const int* etov,
+                         dgc_mesh_t* out);
+/* gmsh MSH 4.1 triangle mesh (src/mesh/gmsh_importer_exporter.h:49-173). */
+int dgc_mesh_read_gmsh(const char* path, dgc_mesh_t* out);
+int dgc_mesh_destroy(dgc_mesh_t mesh);
+int dgc_mesh_info(dgc_mesh_t mesh, int* num_cells, int* num_vertices, double* min_edge);
+/* Per-cell vertex coordinates cx, cy [3K], vertex ids etov [3K] and the twin
+ * half-edge map nb_cell, nb_face [3K] (src/mesh/simple_triangular_mesh_2d.h:116-121). */
+int dgc_mesh_cells(dgc_mesh_t mesh, double* cx, double* cy, int* etov, int* nb_cell, int* nb_face);
+/* Geometry of order `order` (src/simple_discretization_2d.h:84-218). Call once
+ * with bx == by == NULL to learn *num_boundary_nodes. */
+int dgc_mesh_geometry(dgc_mesh_t mesh, int order, double* inv_jac, double* J, double* face_J,
+                      double* nx, double* ny, int* iface_cell, int* iface_face,
+                      int* num_boundary_nodes, double* bx, double* by);
+/* Physical DOF coordinates, node-major (node_major = 1) or cell-major. */
+int dgc_mesh_dof_coords(dgc_mesh_t mesh, int order, int node_major, double* x, double* y);
+
+/* ---- device discretization ----------------------------------------------
+ * Replaces create_simple_discretization_2d_on_device /
+ * destroy_simple_discretization_2d_on_device (src/d_simple_discretization_2d.cuh:89-152):
+ * same argument list, but the library owns the device copies and returns a
+ * handle. Orders 1..dgc_max_order() are accepted (the reference: 1..6). */
+int dgc_disc2d_create(dgc_ctx_t ctx, int num_cells, int order, const int* face0_nodes,
+                      const int* face1_nodes, const int* face2_nodes, const double* inv_jac,
+                      const double* J, const double* face_J, const int* iface_cell,
+                      const int* iface_face, int num_boundary_nodes, const double* bx,
+                      const double* by, const double* nx, const double* ny, dgc_disc_t* out);
+int dgc_disc2d_destroy(dgc_disc_t disc);
+
+/* ---- semi-discrete operators (device objects) ----------------------------
+ * Replace create_device_object / destroy_device_object of
+ * examples/maxwell_2d/k_maxwell_2d.cu:45-116 and
+ * examples/advection_2d/k_advection_2d.cu:45-126. Dr, Ds [Np*Np] and lift
+ * [Np*3Nfp] are the weak-form matrices; they are copied to device memory
+ * (no per-process __constant__ symbols, so several orders can coexist). */
+int dgc_maxwell_tm_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                          dgc_op_t* out);
+/* a = (ax, ay); bc 0: analytic inflow sin(pi(x+1-t)) sin(pi(y+1-t))
+ * (examples/advection_2d/d_advection_2d.cuh:86-91), bc 1: zero inflow.
+ * ax = ay = 1 with bc 0 is the reference problem. */
+int dgc_advection_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                         double ax, double ay, int bc, dgc_op_t* out);
+int dgc_op_destroy(dgc_op_t op);
+int dgc_op_num_fields(dgc_op_t op);
+
+/* out = L_h(in, t): one evaluation of the semi-discrete operator.
+ * Replaces device_SemiDiscOp_wrapper::operator() (src/device_SemiDiscOp_wrapper.cuh:45-58).
+ * in/out: arrays of F device pointers (F = 3 for Maxwell: Hx, Hy, Ez), each K*Np. */
+int dgc_op_rhs(dgc_op_t op, const double* const* in, size_t size, double t, double* const* out);
+
+/* out = a * x + y over n doubles (src/k_axpy.cuh:38-59). */
+int dgc_axpy(dgc_ctx_t ctx, double a, const double* x, size_t n, const double* y, double* out);
+
+/* One classical RK4 step, fused: 4 kernels, 17 state passes.
+ * Replaces rk4_on_device (examples/maxwell_2d/k_maxwell_2d.cu:98-111) driving
+ * dgc::rk4 (src/explicit_runge_kutta.h:33-51). inout: F device pointers;
+ * wk: 5*F device pointers (wk[w*F + f], w = 0..4), caller-owned like wk0..wk4. */
+int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, double dt,
+                 double* const* wk);
+
+/* Time loop t0 -> t_end with step dt, the last step truncated to land on
+ * t_end (examples/maxwell_2d/main.cpp:199-209). Full steps are replayed from a
+ * captured hipGraph. *t_out / *steps_out report the final time and the step
+ * count (either may be NULL). */
+int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, double t_end, double dt,
+                double* const* wk, double* t_out, int* steps_out);
+
+/* Fixed number of full steps (examples/advection_2d/main.cpp:135-152), graph-replayed. */
+int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, double dt, int nsteps,
+                  double* const* wk);
+
+/* Same as dgc_rk4_steps, bracketed by HIP events on the context stream;
+ * blocks until done and returns the elapsed milliseconds in *ms
+ * (measurement entry point; no reference counterpart). */
+int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
+                        int nsteps, double* const* wk, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGC_HIP_H */

@@ -1,0 +1,102 @@
+"""ctypes access to the oracle (oracle/liboracle.so) -- test infrastructure.
+
+The oracle is the CPU restatement of the reference's serial path; tests use
+it as the checker of the HIP path, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+_L = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError("oracle not built: run oracle/build_ref.sh (or __graft_entry__.build())")
+        L = C.CDLL(_PATH)
+        L.oracle_maxwell_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
+                                         C.c_double, _dp]
+        L.oracle_maxwell_rhs.restype = None
+        L.oracle_advection_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
+                                           _dp, C.c_double, C.c_double, C.c_int, _dp, C.c_double, _dp]
+        L.oracle_advection_rhs.restype = None
+        L.oracle_laplace_apply.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
+                                           _dp]
+        L.oracle_laplace_apply.restype = None
+        L.oracle_rk4.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp,
+                                 C.c_double, C.c_double, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
+                                 C.c_double]
+        L.oracle_rk4.restype = C.c_int
+        _L = L
+    return _L
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class Problem:
+    """Mesh + operators in the oracle's (reference CPU) conventions."""
+
+    def __init__(self, order, Dr, Ds, L, cx, cy, nb_cell, nb_face, r=None, s=None):
+        self.N = order
+        self.Np = (order + 1) * (order + 2) // 2
+        self.K = len(cx) // 3
+        c = np.ascontiguousarray
+        self.Dr, self.Ds, self.L = c(Dr, np.float64), c(Ds, np.float64), c(L, np.float64)
+        self.cx, self.cy = c(cx, np.float64), c(cy, np.float64)
+        self.nb_cell, self.nb_face = c(nb_cell, np.int32), c(nb_face, np.int32)
+        self.r = c(r if r is not None else np.zeros(self.Np), np.float64)
+        self.s = c(s if s is not None else np.zeros(self.Np), np.float64)
+
+    @classmethod
+    def from_mesh(cls, mesh, order, form="weak", ops=None):
+        import dgc_amd
+
+        ops = ops if ops is not None else dgc_amd.ref_operators(order, form)
+        cells = mesh.cells()
+        return cls(order, ops["Dr"], ops["Ds"], ops["LIFT"], cells["cx"], cells["cy"], cells["nb_cell"],
+                   cells["nb_face"], ops["r"], ops["s"])
+
+    def maxwell_rhs(self, u, t=0.0):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros_like(u)
+        lib().oracle_maxwell_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                 _d(self.cy), _i(self.nb_cell), _i(self.nb_face), _d(u), t, _d(out))
+        return out
+
+    def advection_rhs(self, u, t=0.0, ax=1.0, ay=1.0, bc=0):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros_like(u)
+        lib().oracle_advection_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                   _d(self.cy), _i(self.nb_cell), _i(self.nb_face), _d(self.r),
+                                   _d(self.s), ax, ay, bc, _d(u), t, _d(out))
+        return out
+
+    def laplace(self, u, tau_mode=0):
+        v = np.array(u, dtype=np.float64, copy=True)
+        lib().oracle_laplace_apply(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                   _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, _d(v))
+        return v
+
+    def rk4(self, phys, u, t0, dt, nsteps=0, t_end=-1.0, ax=1.0, ay=1.0, bc=0):
+        v = np.array(u, dtype=np.float64, copy=True)
+        steps = lib().oracle_rk4(phys, self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L),
+                                 _d(self.cx), _d(self.cy), _i(self.nb_cell), _i(self.nb_face),
+                                 _d(self.r), _d(self.s), ax, ay, bc, _d(v), t0, dt, nsteps, t_end)
+        return v, steps

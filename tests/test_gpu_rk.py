@@ -1,0 +1,201 @@
+"""GPU parity of the fused RHS + RK4 path (through the C ABI) against the
+reference's golden vectors and the oracle, FP64: relative L2 <= 1e-12."""
+import numpy as np
+import pytest
+
+import dgc_amd
+from dgc_amd import node_to_cell_major as n2c, cell_to_node_major as c2n
+from oracle_lib import Problem
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12  # FP64 parity bar: relative L2 against the reference CPU path
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def upload_fields(ctx, u_cell, F, K, Np):
+    return [dgc_amd.DeviceArray.from_host(ctx, c2n(f, K, Np)) for f in np.split(u_cell, F)]
+
+
+def download_fields(arrs, K, Np):
+    return np.concatenate([n2c(a.download(), K, Np) for a in arrs])
+
+
+@pytest.fixture(scope="module")
+def c1(ctx, golden):
+    fix = golden("maxwell_c1.npz")
+    N = int(fix["order"])
+    mesh = dgc_amd.Mesh.from_arrays(fix["vx"], fix["vy"], fix["etov"])
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.MaxwellTM(disc)
+    return fix, mesh, disc, op
+
+
+def test_maxwell_rhs_matches_reference_and_oracle(ctx, c1):
+    fix, mesh, disc, op = c1
+    K, Np = disc.K, disc.Np
+    n = K * Np
+    u = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    rng = np.random.default_rng(1)
+    for trial in range(2):
+        if trial == 1:
+            u = rng.standard_normal(3 * n)
+        din = upload_fields(ctx, u, 3, K, Np)
+        dout = [dgc_amd.DeviceArray(ctx, n) for _ in range(3)]
+        op.rhs(din, 0.0, dout)
+        got = download_fields(dout, K, Np)
+        want = Problem.from_mesh(mesh, disc.order).maxwell_rhs(u)
+        assert rel(got, want) < 1e-13, trial
+        if trial == 0:
+            assert rel(got, fix["rhs0"]) < 1e-13
+
+
+def test_maxwell_cavity_t1_matches_reference(ctx, c1):
+    """The headline parity case: K=512, N=4, t = 1 (491 RK4 steps)."""
+    fix, mesh, disc, op = c1
+    K, Np = disc.K, disc.Np
+    n = K * Np
+    u0 = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    state = upload_fields(ctx, u0, 3, K, Np)
+    work = op.new_work()
+    t, steps = op.rk4_run(state, 0.0, 1.0, float(fix["dt"]), work)
+    assert steps == int(fix["steps"]) and t == float(fix["t_final"])
+    got = download_fields(state, K, Np)
+    err = rel(got, fix["final"])
+    print(f"maxwell C1 t=1 rel L2 vs reference = {err:.3e}")
+    assert err < TOL
+    x, y, s2 = fix["x"], fix["y"], np.sqrt(2.0)
+    exact = np.concatenate([-np.sin(np.pi * x) * np.cos(np.pi * y) * np.sin(s2 * np.pi) / s2,
+                            np.cos(np.pi * x) * np.sin(np.pi * y) * np.sin(s2 * np.pi) / s2,
+                            np.sin(np.pi * x) * np.sin(np.pi * y) * np.cos(s2 * np.pi)])
+    assert np.mean((got - exact) ** 2) < 1e-8
+
+
+def test_single_steps_equal_graph_replay(ctx, c1):
+    fix, mesh, disc, op = c1
+    K, Np = disc.K, disc.Np
+    rng = np.random.default_rng(2)
+    u0 = rng.standard_normal(3 * K * Np)
+    a = upload_fields(ctx, u0, 3, K, Np)
+    b = upload_fields(ctx, u0, 3, K, Np)
+    wa, wb = op.new_work(), op.new_work()
+    dt = float(fix["dt"])
+    for i in range(5):
+        op.rk4_step(a, i * dt, dt, wa)
+    op.rk4_steps(b, 0.0, dt, 5, wb)
+    np.testing.assert_array_equal(download_fields(a, K, Np), download_fields(b, K, Np))
+    # and one step against the reference's first step
+    s = upload_fields(ctx, np.concatenate([np.zeros(2 * K * Np),
+                                           np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])]), 3, K, Np)
+    op.rk4_step(s, 0.0, dt, wa)
+    assert rel(download_fields(s, K, Np), fix["step1"]) < 1e-13
+
+
+def test_advection_reference_problem(ctx, golden):
+    """a = (1, 1) with the reference's analytic inflow data (time-dependent BC)."""
+    fix = golden("advection_ref.npz")
+    N = int(fix["order"])
+    mesh = dgc_amd.Mesh.from_arrays(fix["vx"], fix["vy"], fix["etov"])
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.Advection(disc, 1.0, 1.0, dgc_amd.Advection.BC_SINSIN)
+    K, Np = disc.K, disc.Np
+    din = upload_fields(ctx, fix["ic"], 1, K, Np)
+    dout = [dgc_amd.DeviceArray(ctx, K * Np)]
+    op.rhs(din, float(fix["t_rhs"]), dout)
+    assert rel(download_fields(dout, K, Np), fix["rhs"]) < 1e-13
+    state = upload_fields(ctx, fix["ic"], 1, K, Np)
+    work = op.new_work()
+    op.rk4_steps(state, 0.0, float(fix["dt"]), int(fix["nsteps"]), work)
+    assert rel(download_fields(state, K, Np), fix["final"]) < TOL
+    # graph replays driven by the device clock equal explicit-time single steps
+    s2 = upload_fields(ctx, fix["ic"], 1, K, Np)
+    dt = float(fix["dt"])
+    t = 0.0
+    for _ in range(int(fix["nsteps"])):
+        op.rk4_step(s2, t, dt, work)
+        t += dt
+    np.testing.assert_array_equal(download_fields(s2, K, Np), download_fields(state, K, Np))
+
+
+@pytest.mark.parametrize("N", range(1, 8))
+def test_periodic_advection_all_orders_vs_oracle(ctx, N):
+    """Config-2 physics (a = (1, 0.5), periodic) on a 24x24 mesh, 20 steps."""
+    mesh = dgc_amd.Mesh.structured(24, 24, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO)
+    K, Np = disc.K, disc.Np
+    x, y = mesh.dof_coords(N, node_major=False)
+    u0 = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    dt = 0.5 * mesh.min_edge / (np.hypot(1.0, 0.5) * (N + 1) ** 2)
+    state = upload_fields(ctx, u0, 1, K, Np)
+    op.rk4_steps(state, 0.0, dt, 20, op.new_work())
+    got = download_fields(state, K, Np)
+    want, _ = Problem.from_mesh(mesh, N).rk4(1, u0, 0.0, dt, nsteps=20, ax=1.0, ay=0.5, bc=1)
+    assert rel(got, want) < TOL
+
+
+@pytest.mark.parametrize("N", [1, 3, 7])
+def test_maxwell_all_orders_vs_oracle(ctx, N):
+    mesh = dgc_amd.Mesh.structured(10, 10, -1, 1, -1, 1, jitter=0.1, seed=6)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.MaxwellTM(disc)
+    K, Np = disc.K, disc.Np
+    rng = np.random.default_rng(N)
+    u0 = rng.standard_normal(3 * K * Np)
+    dt = 0.5 * mesh.min_edge / (N + 1) ** 2
+    state = upload_fields(ctx, u0, 3, K, Np)
+    op.rk4_steps(state, 0.0, dt, 10, op.new_work())
+    want, _ = Problem.from_mesh(mesh, N).rk4(0, u0, 0.0, dt, nsteps=10)
+    assert rel(download_fields(state, K, Np), want) < TOL
+
+
+def test_tail_tile_and_odd_sizes(ctx):
+    """K not a multiple of the 64-cell tile."""
+    mesh = dgc_amd.Mesh.structured(7, 5, -1, 1, -1, 1, jitter=0.1, seed=8)  # K = 70
+    for N in (2, 5):
+        disc = dgc_amd.Discretization(ctx, mesh, N)
+        op = dgc_amd.MaxwellTM(disc)
+        K, Np = disc.K, disc.Np
+        u = np.random.default_rng(0).standard_normal(3 * K * Np)
+        dout = [dgc_amd.DeviceArray(ctx, K * Np) for _ in range(3)]
+        op.rhs(upload_fields(ctx, u, 3, K, Np), 0.0, dout)
+        assert rel(download_fields(dout, K, Np), Problem.from_mesh(mesh, N).maxwell_rhs(u)) < 1e-13
+
+
+def test_periodic_mass_conservation_full_config2_size(ctx):
+    """Config-2 size K = 131072 at N = 3: mass is conserved to rounding."""
+    N = 3
+    mesh = dgc_amd.Mesh.structured(256, 256, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO)
+    K, Np = disc.K, disc.Np
+    x, y = mesh.dof_coords(N, node_major=True)
+    u0 = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    M = dgc_amd.ref_operators(N)["M"]
+    J = disc.geometry["J"]
+
+    def mass(u):
+        return float(np.sum(J * (np.ones(Np) @ M @ u.reshape(Np, K))))
+
+    state = [dgc_amd.DeviceArray.from_host(ctx, u0)]
+    dt = 0.5 * mesh.min_edge / (np.hypot(1.0, 0.5) * (N + 1) ** 2)
+    op.rk4_steps(state, 0.0, dt, 50, op.new_work())
+    u = state[0].download()
+    assert np.all(np.isfinite(u))
+    assert abs(mass(u) - mass(u0)) < 1e-12 * abs(mass(u0))
+    assert np.sum(u**2) <= np.sum(u0**2) * 1.0001  # no growth
+
+
+def test_bad_arguments_are_errors(ctx, c1):
+    fix, mesh, disc, op = c1
+    a = [dgc_amd.DeviceArray(ctx, disc.size) for _ in range(3)]
+    with pytest.raises(dgc_amd.DGCError):
+        dgc_amd.lib()  # loaded
+        from dgc_amd.core import check, _ptrs
+
+        check(dgc_amd.lib().dgc_op_rhs(op._h, _ptrs([x.ptr for x in a]), disc.size + 1, 0.0,
+                                       _ptrs([x.ptr for x in a])))
+    with pytest.raises(dgc_amd.DGCError):
+        op.rk4_run(a, 0.0, 1.0, -1.0, op.new_work())

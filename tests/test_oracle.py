@@ -1,0 +1,146 @@
+"""Pins the oracle (the CPU restatement of the reference's serial path) to
+the golden vectors written by the compiled reference, and to properties the
+mathematics fixes where the reference has no counterpart."""
+import numpy as np
+import pytest
+
+import dgc_amd
+from oracle_lib import Problem
+
+
+def _ref_ops(golden, N, form):
+    g = golden("refelem.npz")
+    Np = (N + 1) * (N + 2) // 2
+    sfx = "w" if form == "weak" else "s"
+    return dict(Dr=g[f"N{N}_Dr_{sfx}"].reshape(Np, Np), Ds=g[f"N{N}_Ds_{sfx}"].reshape(Np, Np),
+                LIFT=g[f"N{N}_L_{sfx}"].reshape(Np, -1), r=g[f"N{N}_r"], s=g[f"N{N}_s"])
+
+
+def _problem(golden, fix, form="weak", prefix=""):
+    N = int(fix[prefix + "order"])
+    m = dgc_amd.Mesh.from_arrays(fix[prefix + "vx"], fix[prefix + "vy"], fix[prefix + "etov"])
+    return Problem.from_mesh(m, N, ops=_ref_ops(golden, N, form)), m
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_maxwell_rhs_matches_reference(golden):
+    fix = golden("maxwell_c1.npz")
+    P, _ = _problem(golden, fix)
+    n = P.K * P.Np
+    u = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    assert rel(P.maxwell_rhs(u), fix["rhs0"]) < 1e-14
+
+
+def test_maxwell_rk4_to_t1_matches_reference(golden):
+    fix = golden("maxwell_c1.npz")
+    P, _ = _problem(golden, fix)
+    n = P.K * P.Np
+    u0 = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    u1, steps = P.rk4(0, u0, 0.0, float(fix["dt"]), nsteps=1)
+    assert rel(u1, fix["step1"]) < 1e-14
+    uT, steps = P.rk4(0, u0, 0.0, float(fix["dt"]), t_end=1.0)
+    assert steps == int(fix["steps"])
+    assert rel(uT, fix["final"]) < 1e-13
+    # and the reference itself is close to the analytic cavity mode
+    x, y, t = fix["x"], fix["y"], 1.0
+    s2 = np.sqrt(2.0)
+    exact = np.concatenate([-np.sin(np.pi * x) * np.cos(np.pi * y) * np.sin(s2 * np.pi * t) / s2,
+                            np.cos(np.pi * x) * np.sin(np.pi * y) * np.sin(s2 * np.pi * t) / s2,
+                            np.sin(np.pi * x) * np.sin(np.pi * y) * np.cos(s2 * np.pi * t)])
+    assert np.mean((uT - exact) ** 2) < 1e-8
+
+
+def test_advection_matches_reference(golden):
+    fix = golden("advection_ref.npz")
+    P, _ = _problem(golden, fix)
+    assert rel(P.advection_rhs(fix["ic"], float(fix["t_rhs"]), 1.0, 1.0, 0), fix["rhs"]) < 1e-14
+    u, _ = P.rk4(1, fix["ic"], 0.0, float(fix["dt"]), nsteps=int(fix["nsteps"]), ax=1.0, ay=1.0, bc=0)
+    assert rel(u, fix["final"]) < 1e-14
+
+
+def test_laplace_matches_reference(golden):
+    fix = golden("laplace_ref.npz")
+    P, _ = _problem(golden, fix, form="strong")
+    for vin, vout in zip(fix["vin"], fix["vout"]):
+        assert rel(P.laplace(vin, 0), vout) < 1e-14
+    P2, _ = _problem(golden, fix, form="strong", prefix="coarse_")
+    n = P2.K * P2.Np
+    A = np.stack([P2.laplace(e, 0) for e in np.eye(n)], axis=1)
+    np.testing.assert_allclose(A, fix["coarse_A"], atol=1e-12 * np.abs(A).max(), rtol=0)
+
+
+def _mass_blocks(mesh, N):
+    w = dgc_amd.ref_operators(N, "weak")
+    J = mesh.geometry(N)["J"]
+    return [Jc * w["M"] for Jc in J]
+
+
+@pytest.mark.parametrize("tau_mode", [0, 1])
+def test_laplace_minus_MA_symmetric_positive_definite(tau_mode):
+    N = 2
+    m = dgc_amd.Mesh.structured(4, 4, -1, 1, -1, 1, jitter=0.1, seed=9)
+    P = Problem.from_mesh(m, N, form="strong")
+    n = P.K * P.Np
+    A = np.stack([P.laplace(e, tau_mode) for e in np.eye(n)], axis=1)
+    from scipy.linalg import block_diag
+
+    MA = -block_diag(*_mass_blocks(m, N)) @ A
+    assert np.abs(MA - MA.T).max() < 1e-12 * np.abs(MA).max()
+    ev = np.linalg.eigvalsh(0.5 * (MA + MA.T))
+    assert ev.min() > 0
+
+
+def test_laplace_planted_mistake_is_caught(golden):
+    """A dropped penalty (tau = 0) must break the golden comparison; this
+    guards the check itself against passing vacuously."""
+    fix = golden("laplace_ref.npz")
+    P, _ = _problem(golden, fix, form="strong")
+    good = P.laplace(fix["vin"][0], 0)
+    bad = P.laplace(fix["vin"][0], 1)  # different penalty: must not match
+    assert rel(good, fix["vout"][0]) < 1e-14
+    assert rel(bad, fix["vout"][0]) > 1e-3
+
+
+@pytest.mark.parametrize("tau_mode", [0, 1])
+def test_laplace_polynomial_exactness_in_interior(tau_mode):
+    N = 3
+    m = dgc_amd.Mesh.structured(8, 8, -1, 1, -1, 1, jitter=0.1, seed=4)
+    P = Problem.from_mesh(m, N, form="strong")
+    x, y = m.dof_coords(N, node_major=False)
+    lap = P.laplace(x**2 + 3 * y**2 - x * y, tau_mode).reshape(P.K, P.Np)
+    # cells two rings away from the boundary see no boundary penalty
+    nb = P.nb_cell.reshape(-1, 3)
+    bnd = {c for c in range(P.K) if (nb[c] == c).any()}
+    ring = bnd | {int(o) for c in bnd for o in nb[c]}
+    inner = [c for c in range(P.K) if c not in ring and not any(int(o) in ring for o in nb[c])]
+    assert len(inner) > 10
+    np.testing.assert_allclose(lap[inner], 8.0, atol=1e-9)
+
+
+def test_maxwell_energy_nonincreasing_and_advection_conservation():
+    N = 3
+    w = dgc_amd.ref_operators(N, "weak")
+    rng = np.random.default_rng(0)
+    m = dgc_amd.Mesh.structured(6, 6, -1, 1, -1, 1, jitter=0.1, seed=2)
+    P = Problem.from_mesh(m, N, ops=w)
+    Mb = _mass_blocks(m, N)
+    u = rng.standard_normal(3 * P.K * P.Np)
+    du = P.maxwell_rhs(u)
+    dE = sum(f.reshape(P.K, P.Np)[c] @ Mb[c] @ g.reshape(P.K, P.Np)[c]
+             for f, g in zip(np.split(u, 3), np.split(du, 3)) for c in range(P.K))
+    assert dE < 0  # upwind dissipation (and PEC is lossless)
+    mp = dgc_amd.Mesh.structured(6, 6, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    Pp = Problem.from_mesh(mp, N, ops=w)
+    Mp = _mass_blocks(mp, N)
+    v = rng.standard_normal(Pp.K * Pp.Np)
+    dv = Pp.advection_rhs(v, 0.0, 1.0, 0.5, 1).reshape(Pp.K, Pp.Np)
+    one = np.ones(Pp.Np)
+    mass_rate = sum(one @ Mp[c] @ dv[c] for c in range(Pp.K))
+    assert abs(mass_rate) < 1e-10 * np.abs(dv).max()
+    vv = v.reshape(Pp.K, Pp.Np)
+    assert sum(vv[c] @ Mp[c] @ dv[c] for c in range(Pp.K)) < 0
+    const = Pp.advection_rhs(np.ones(Pp.K * Pp.Np), 0.0, 1.0, 0.5, 1)
+    assert np.abs(const).max() < 1e-10
