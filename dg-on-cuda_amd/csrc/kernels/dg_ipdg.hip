@@ -1,0 +1,505 @@
+// IPDG (BR1-type interior penalty) Laplacian and the CG building blocks.
+//
+// Replaces src/laplace_2d.h:130-251 (the two-pass apply_homogeneous) and its
+// device counterpart examples/poisson_2d/d_poisson_2d.cuh:55-172 launched
+// through src/kernel_adapter.cuh:77-82. Boundary data follow the CPU path
+// (examples/poisson_2d/poisson_2d.h:73-79: Dirichlet everywhere, u+ = -u-,
+// dq.n = 0 at the wall), not the GPU example's affine variant.
+//
+//   pass 1: q = grad u - LIFT(n [u] / 2),            [u] = u- - u+ (2u- at the wall)
+//   pass 2: Lap u = div q - LIFT((n.[q] + 2 tau [u]) / 2), n.[q] = 0 at the wall
+//
+// Same tiling as the RK kernels: 64 cells per workgroup (lane = cell), node
+// rows in LDS, operator rows wave-uniform (scalar loads). Templated on the
+// value type so the same code provides the FP64 operator and the FP32 inner
+// operator of the mixed-precision solver. Dot products are reduced per
+// workgroup into a partial-sum array (fixed order, no atomics) and finished
+// by a one-block kernel, so every solve is bit-reproducible.
+#include <algorithm>
+
+#include "dg_ipdg.hpp"
+
+namespace dgc {
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  // wave reduction then across the 4 waves, fixed order
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kWaves; ++w) s += red[w];
+  }
+  return s;
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
+  using Tr = Tri<N>;
+  constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp;
+  constexpr int RPW = (Np + kWaves - 1) / kWaves;
+  constexpr int Npad = RPW * kWaves;
+  __shared__ T us[Np][kTile];
+  __shared__ T lx[NF][kTile];
+  __shared__ T ly[NF][kTile];
+  const int K = p.K;
+  const int lane = threadIdx.x & (kTile - 1);
+  const int wave = uniform(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * kTile;
+  const int c = c0 + lane;
+  const bool valid = c < K;
+  if (p.scal && p.scal[kDone] != T(0)) return;
+
+  for (int j = wave; j < Np; j += kWaves) us[j][lane] = valid ? p.u[(size_t)j * K + c] : T(0);
+  __syncthreads();
+
+  for (int m = wave; m < NF; m += kWaves) {
+    const int e = m / Nfp, d = m - e * Nfp;
+    const T a = us[Tr::fnode(e, d)][lane];
+    T du = 0, nx = 0, ny = 0, fs = 0;
+    if (valid) {
+      nx = p.geo[(size_t)(kNx0 + e) * K + c];
+      ny = p.geo[(size_t)(kNy0 + e) * K + c];
+      fs = p.geo[(size_t)(kFs0 + e) * K + c];
+      const int code = p.nbr[(size_t)e * K + c];
+      const int nf = code & 3, idx = code >> 2;
+      if (nf == kBoundaryCode) {
+        du = T(2) * a;
+      } else {
+        const int nn = Tr::fnode(nf, Nfp - 1 - d);
+        const int l = idx - c0;
+        const T b = ((unsigned)l < (unsigned)kTile) ? us[nn][l] : p.u[(size_t)nn * K + idx];
+        du = a - b;
+      }
+      p.du[(size_t)m * K + c] = du;
+    }
+    lx[m][lane] = nx * du * fs / T(2);
+    ly[m][lane] = ny * du * fs / T(2);
+  }
+  __syncthreads();
+
+  const int j0 = wave * RPW;
+  T dr[RPW], ds[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) dr[i] = ds[i] = 0;
+#pragma unroll
+  for (int k = 0; k < Np; ++k) {
+    const T uk = us[k][lane];
+    const T* a = p.DrT + k * Npad + j0;
+    const T* b = p.DsT + k * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      dr[i] = fma(a[i], uk, dr[i]);
+      ds[i] = fma(b[i], uk, ds[i]);
+    }
+  }
+  T rx = 0, ry = 0, sx = 0, sy = 0;
+  if (valid) {
+    rx = p.geo[(size_t)kRx * K + c];
+    ry = p.geo[(size_t)kRy * K + c];
+    sx = p.geo[(size_t)kSx * K + c];
+    sy = p.geo[(size_t)kSy * K + c];
+  }
+  T qx[RPW], qy[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    qx[i] = rx * dr[i] + sx * ds[i];
+    qy[i] = ry * dr[i] + sy * ds[i];
+  }
+#pragma unroll
+  for (int m = 0; m < NF; ++m) {
+    const T fx = lx[m][lane], fy = ly[m][lane];
+    const T* l = p.LT + m * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      qx[i] = fma(-l[i], fx, qx[i]);
+      qy[i] = fma(-l[i], fy, qy[i]);
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = j0 + i;
+    if (j >= Np) break;
+    p.qx[(size_t)j * K + c] = qx[i];
+    p.qy[(size_t)j * K + c] = qy[i];
+  }
+}
+
+// mode kLaplace: out = Lap u; mode kSpd: out = -J M Lap u (the SPD form),
+// optionally with partial[block] = sum over the tile of dotv . out.
+template <int N, typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
+  using Tr = Tri<N>;
+  constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp;
+  constexpr int RPW = (Np + kWaves - 1) / kWaves;
+  constexpr int Npad = RPW * kWaves;
+  __shared__ T qxs[Np][kTile];
+  __shared__ T qys[Np][kTile];
+  __shared__ T lf[NF][kTile];
+  __shared__ T red[kWaves];
+  const int K = p.K;
+  const int lane = threadIdx.x & (kTile - 1);
+  const int wave = uniform(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * kTile;
+  const int c = c0 + lane;
+  const bool valid = c < K;
+  if (p.scal && p.scal[kDone] != T(0)) return;
+
+  for (int j = wave; j < Np; j += kWaves) {
+    qxs[j][lane] = valid ? p.qx[(size_t)j * K + c] : T(0);
+    qys[j][lane] = valid ? p.qy[(size_t)j * K + c] : T(0);
+  }
+  __syncthreads();
+
+  for (int m = wave; m < NF; m += kWaves) {
+    const int e = m / Nfp, d = m - e * Nfp;
+    const int own = Tr::fnode(e, d);
+    T v = 0;
+    if (valid) {
+      const T nx = p.geo[(size_t)(kNx0 + e) * K + c];
+      const T ny = p.geo[(size_t)(kNy0 + e) * K + c];
+      const T fs = p.geo[(size_t)(kFs0 + e) * K + c];
+      const T tau = p.tau[(size_t)e * K + c];
+      const T du = p.du[(size_t)m * K + c];
+      const int code = p.nbr[(size_t)e * K + c];
+      const int nf = code & 3, idx = code >> 2;
+      T dqn = 0;
+      if (nf != kBoundaryCode) {
+        const int nn = Tr::fnode(nf, Nfp - 1 - d);
+        const int l = idx - c0;
+        T bx, by;
+        if ((unsigned)l < (unsigned)kTile) {
+          bx = qxs[nn][l];
+          by = qys[nn][l];
+        } else {
+          bx = p.qx[(size_t)nn * K + idx];
+          by = p.qy[(size_t)nn * K + idx];
+        }
+        dqn = (qxs[own][lane] - bx) * nx + (qys[own][lane] - by) * ny;
+      }
+      v = (dqn + T(2) * (tau * du)) * fs / T(2);
+    }
+    lf[m][lane] = v;
+  }
+  __syncthreads();
+
+  const int j0 = wave * RPW;
+  T ax[RPW], bx[RPW], ay[RPW], by[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) ax[i] = bx[i] = ay[i] = by[i] = 0;
+#pragma unroll
+  for (int k = 0; k < Np; ++k) {
+    const T qk = qxs[k][lane], rk = qys[k][lane];
+    const T* a = p.DrT + k * Npad + j0;
+    const T* b = p.DsT + k * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      ax[i] = fma(a[i], qk, ax[i]);
+      bx[i] = fma(b[i], qk, bx[i]);
+      ay[i] = fma(a[i], rk, ay[i]);
+      by[i] = fma(b[i], rk, by[i]);
+    }
+  }
+  T rx = 0, ry = 0, sx = 0, sy = 0;
+  if (valid) {
+    rx = p.geo[(size_t)kRx * K + c];
+    ry = p.geo[(size_t)kRy * K + c];
+    sx = p.geo[(size_t)kSx * K + c];
+    sy = p.geo[(size_t)kSy * K + c];
+  }
+  T lap[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) lap[i] = (rx * ax[i] + sx * bx[i]) + (ry * ay[i] + sy * by[i]);
+#pragma unroll
+  for (int m = 0; m < NF; ++m) {
+    const T f = lf[m][lane];
+    const T* l = p.LT + m * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) lap[i] = fma(-l[i], f, lap[i]);
+  }
+
+  if (MODE == kLaplace) {
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int j = j0 + i;
+        if (j < Np) p.out[(size_t)j * K + c] = lap[i];
+      }
+    }
+    return;
+  }
+  // SPD form: out = -J M lap. Rows of lap go through LDS (reusing qxs).
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = j0 + i;
+    if (j < Np) qxs[j][lane] = lap[i];
+  }
+  __syncthreads();
+  const T Jc = valid ? p.J[c] : T(0);
+  T o[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) o[i] = 0;
+#pragma unroll
+  for (int k = 0; k < Np; ++k) {
+    const T v = qxs[k][lane];
+    const T* mm = p.MT + k * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) o[i] = fma(mm[i], v, o[i]);
+  }
+  T dot = 0;
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int j = j0 + i;
+      if (j < Np) {
+        const T val = -Jc * o[i];
+        p.out[(size_t)j * K + c] = val;
+        if (p.dotv) dot += p.dotv[(size_t)j * K + c] * val;
+      }
+    }
+  }
+  if (p.partial) {
+    const T s = block_sum(dot, red);
+    if (threadIdx.x == 0) p.partial[blockIdx.x] = s;
+  }
+}
+
+// out = J M f (the mass-weighted right-hand side)
+template <int N, typename T>
+__global__ __launch_bounds__(kBlock) void ipdg_mass(const IpdgParams<T> p) {
+  using Tr = Tri<N>;
+  constexpr int Np = Tr::Np;
+  constexpr int RPW = (Np + kWaves - 1) / kWaves;
+  constexpr int Npad = RPW * kWaves;
+  __shared__ T fs[Np][kTile];
+  const int K = p.K;
+  const int lane = threadIdx.x & (kTile - 1);
+  const int wave = uniform(threadIdx.x >> 6);
+  const int c = blockIdx.x * kTile + lane;
+  const bool valid = c < K;
+  for (int j = wave; j < Np; j += kWaves) fs[j][lane] = valid ? p.u[(size_t)j * K + c] : T(0);
+  __syncthreads();
+  const int j0 = wave * RPW;
+  T o[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) o[i] = 0;
+#pragma unroll
+  for (int k = 0; k < Np; ++k) {
+    const T v = fs[k][lane];
+    const T* mm = p.MT + k * Npad + j0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) o[i] = fma(mm[i], v, o[i]);
+  }
+  if (!valid) return;
+  const T Jc = p.J[c];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = j0 + i;
+    if (j < Np) p.out[(size_t)j * K + c] = Jc * o[i];
+  }
+}
+
+// CG update with block-Jacobi preconditioning, one tile of cells:
+//   x += alpha p;  r -= alpha q;  z = P_c r_c (P: per-cell Np x Np, entry
+//   (j,k) of cell c at P[(j*Np + k) * K + c]; identity if P is null);
+//   partials: r.z and r.r.
+template <int N, typename T>
+__global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
+  using Tr = Tri<N>;
+  constexpr int Np = Tr::Np;
+  __shared__ T rs[Np][kTile];
+  __shared__ T red[kWaves];
+  const int K = p.K;
+  const int lane = threadIdx.x & (kTile - 1);
+  const int wave = uniform(threadIdx.x >> 6);
+  const int c = blockIdx.x * kTile + lane;
+  const bool valid = c < K;
+  const T alpha = p.scal[kAlpha];
+  const bool skip = p.scal[kDone] != T(0);
+  T rr = 0, rz = 0;
+  for (int j = wave; j < Np; j += kWaves) {
+    T r = 0;
+    if (valid) {
+      const size_t idx = (size_t)j * K + c;
+      r = p.r[idx];
+      if (!skip) {
+        p.x[idx] = p.x[idx] + alpha * p.p[idx];
+        r = r - alpha * p.q[idx];
+        p.r[idx] = r;
+      }
+    }
+    rs[j][lane] = r;
+    rr += r * r;
+  }
+  __syncthreads();
+  for (int j = wave; j < Np; j += kWaves) {
+    T z = rs[j][lane];
+    if (p.P && valid) {
+      z = 0;
+#pragma unroll
+      for (int k = 0; k < Np; ++k) z = fma(p.P[((size_t)j * Np + k) * K + c], rs[k][lane], z);
+    }
+    if (valid) p.z[(size_t)j * K + c] = z;
+    rz += rs[j][lane] * z;
+  }
+  T s1 = block_sum(rz, red);
+  __syncthreads();
+  T s2 = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    p.partial[2 * blockIdx.x] = s1;
+    p.partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// p = z + beta p (beta = 0 on the first iteration)
+template <typename T>
+__global__ void cg_direction(const T* __restrict__ z, T* __restrict__ pv, const T* __restrict__ scal,
+                             size_t n) {
+  if (scal[kDone] != T(0)) return;
+  const T beta = scal[kBeta];
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) pv[i] = z[i] + beta * pv[i];
+}
+
+// Finish a reduction of `nparts` partials (stride `stride`, offset `off`),
+// then update the CG scalars:
+//   which == kAfterApply:  alpha = rz / (p.q)
+//   which == kAfterUpdate: beta = rz_new / rz, rz = rz_new, rr = r.r,
+//                          history[it] = rr, done if rr <= tol2 or it+1 >= maxit
+template <typename T>
+__global__ void cg_scalars(const T* __restrict__ partial, int nparts, int which, T* scal,
+                           double* history) {
+  __shared__ double sh[2][1024];
+  double a = 0.0, b = 0.0;
+  const int width = which == kAfterApply ? 1 : 2;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    a += double(partial[width * i]);
+    if (width == 2) b += double(partial[width * i + 1]);
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (which == kInit) {  // rz = r.z, rr = r.r, beta = 0; clears the done flag
+    scal[kRz] = T(sh[0][0]);
+    scal[kRr] = T(sh[1][0]);
+    scal[kBeta] = T(0);
+    if (history) history[0] = sh[1][0];
+    scal[kDone] = T(sh[1][0] <= double(scal[kTol2]) ? 1 : 0);
+    return;
+  }
+  if (scal[kDone] != T(0)) return;
+  if (which == kAfterApply) {
+    scal[kAlpha] = T(double(scal[kRz]) / sh[0][0]);
+  } else if (which == kAfterUpdate) {
+    const double rz = sh[0][0], rr = sh[1][0];
+    scal[kBeta] = T(rz / double(scal[kRz]));
+    scal[kRz] = T(rz);
+    scal[kRr] = T(rr);
+    const int it = int(scal[kIter]);
+    if (history) history[it + 1] = rr;  // history[0] holds the initial residual
+    scal[kIter] = T(it + 1);
+    if (rr <= double(scal[kTol2]) || it + 1 >= int(scal[kMaxit])) scal[kDone] = T(1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T, int N>
+hipError_t launch_apply_n(const IpdgParams<T>& p, int mode, hipStream_t s) {
+  const int grid = (p.K + kTile - 1) / kTile;
+  if (mode != kPhaseTwo) hipLaunchKernelGGL((ipdg_pass1<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
+  if (mode == kLaplace || mode == kPhaseTwo)
+    hipLaunchKernelGGL((ipdg_pass2<N, T, kLaplace>), dim3(grid), dim3(kBlock), 0, s, p);
+  else if (mode == kSpd)
+    hipLaunchKernelGGL((ipdg_pass2<N, T, kSpd>), dim3(grid), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t launch_mass_n(const IpdgParams<T>& p, hipStream_t s) {
+  const int grid = (p.K + kTile - 1) / kTile;
+  hipLaunchKernelGGL((ipdg_mass<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t launch_update_n(const CgParams<T>& p, hipStream_t s) {
+  const int grid = (p.K + kTile - 1) / kTile;
+  hipLaunchKernelGGL((cg_update<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define DGC_ORDER_SWITCH(order, CALL) \
+  switch (order) {                    \
+    case 1: return CALL(1);           \
+    case 2: return CALL(2);           \
+    case 3: return CALL(3);           \
+    case 4: return CALL(4);           \
+    case 5: return CALL(5);           \
+    case 6: return CALL(6);           \
+    case 7: return CALL(7);           \
+    default: return hipErrorInvalidValue; \
+  }
+
+template <typename T>
+hipError_t ipdg_apply(int order, const IpdgParams<T>& p, int mode, hipStream_t s) {
+#define CALL(n) launch_apply_n<T, n>(p, mode, s)
+  DGC_ORDER_SWITCH(order, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t ipdg_mass_apply(int order, const IpdgParams<T>& p, hipStream_t s) {
+#define CALL(n) launch_mass_n<T, n>(p, s)
+  DGC_ORDER_SWITCH(order, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t cg_update_launch(int order, const CgParams<T>& p, hipStream_t s) {
+#define CALL(n) launch_update_n<T, n>(p, s)
+  DGC_ORDER_SWITCH(order, CALL)
+#undef CALL
+}
+
+template <typename T>
+hipError_t cg_direction_launch(const T* z, T* pv, const T* scal, size_t n, hipStream_t s) {
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL((cg_direction<T>), dim3(grid), dim3(256), 0, s, z, pv, scal, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cg_scalars_launch(const T* partial, int nparts, int which, T* scal, double* history,
+                             hipStream_t s) {
+  hipLaunchKernelGGL((cg_scalars<T>), dim3(1), dim3(1024), 0, s, partial, nparts, which, scal, history);
+  return hipGetLastError();
+}
+
+template hipError_t ipdg_apply<double>(int, const IpdgParams<double>&, int, hipStream_t);
+template hipError_t ipdg_apply<float>(int, const IpdgParams<float>&, int, hipStream_t);
+template hipError_t ipdg_mass_apply<double>(int, const IpdgParams<double>&, hipStream_t);
+template hipError_t ipdg_mass_apply<float>(int, const IpdgParams<float>&, hipStream_t);
+template hipError_t cg_update_launch<double>(int, const CgParams<double>&, hipStream_t);
+template hipError_t cg_update_launch<float>(int, const CgParams<float>&, hipStream_t);
+template hipError_t cg_direction_launch<double>(const double*, double*, const double*, size_t, hipStream_t);
+template hipError_t cg_direction_launch<float>(const float*, float*, const float*, size_t, hipStream_t);
+template hipError_t cg_scalars_launch<double>(const double*, int, int, double*, double*, hipStream_t);
+template hipError_t cg_scalars_launch<float>(const float*, int, int, float*, double*, hipStream_t);
+
+}  // namespace dgc

@@ -379,8 +379,9 @@ int dgc_axpy(dgc_ctx_t ctx, double a, const double* x, size_t n, const double* y
 
 // One fused RK4 step. wk mapping: wk0 = stage accumulator (k1 + 2 k2),
 // wk1 / wk2 = stage inputs (ping-pong), wk3 = k3. wk4 is unused.
+// With `evs` (8 events) each stage launch is bracketed by evs[2s], evs[2s+1].
 static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, double* const* wk,
-                        bool use_clock) {
+                        bool use_clock, hipEvent_t* evs = nullptr) {
   const int F = op->F;
   dgc::StageParams p = base_params(op);
   for (int f = 0; f < F; ++f) {
@@ -408,8 +409,11 @@ static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, dou
       p.in[f] = s.in_buf < 0 ? u[f] : wk[s.in_buf * F + f];
       p.out[f] = s.out_buf < 0 ? u[f] : wk[s.out_buf * F + f];
     }
-    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, p, op->disc->ctx->stream);
+    hipStream_t st = op->disc->ctx->stream;
+    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1)], st));
+    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, p, st);
     if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
+    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1) + 1], st));
   }
   return DGC_OK;
 }
@@ -521,6 +525,33 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (err != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("timed replay: ") + hipGetErrorString(err));
+  return DGC_OK;
+}
+
+int dgc_rk4_profile_stages(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
+                           int nsteps, double* const* wk, float* stage_ms) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_rk4_profile_stages");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(nsteps > 0 && stage_ms != nullptr, "dgc_rk4_profile_stages: need nsteps > 0 and stage_ms");
+  hipEvent_t ev[8];
+  for (int i = 0; i < 8; ++i) DGC_HIP_TRY(hipEventCreate(&ev[i]));
+  double acc[4] = {0, 0, 0, 0};
+  double t = t0;
+  for (int n = 0; n < nsteps && rc == DGC_OK; ++n) {
+    rc = enqueue_step(op, inout, t, dt, wk, false, ev);
+    if (rc != DGC_OK) break;
+    hipError_t e = hipEventSynchronize(ev[7]);
+    for (int sidx = 0; sidx < 4 && e == hipSuccess; ++sidx) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev[2 * sidx], ev[2 * sidx + 1]);
+      acc[sidx] += ms;
+    }
+    if (e != hipSuccess) rc = dgc_set_error(DGC_ERR_HIP, std::string("stage events: ") + hipGetErrorString(e));
+    t += dt;
+  }
+  for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+  if (rc != DGC_OK) return rc;
+  for (int sidx = 0; sidx < 4; ++sidx) stage_ms[sidx] = float(acc[sidx] / nsteps);
   return DGC_OK;
 }
 

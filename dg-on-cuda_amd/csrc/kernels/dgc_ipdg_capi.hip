@@ -1,0 +1,659 @@
+// C ABI of the IPDG operator and its solvers (include/dgc_hip.h):
+// apply (the laplace_2d replacement), block-Jacobi PCG in FP64 and the
+// FP32-inner / FP64-outer iterative-refinement variant.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../../include/dgc_hip.h"
+#include "../host/refelem.hpp"
+#include "dg_internal.hpp"
+#include "dg_ipdg.hpp"
+#include "dgc_ipdg_internal.hpp"
+
+#define DGC_HIP_TRY(call)                                                                     \
+  do {                                                                                        \
+    hipError_t err_ = (call);                                                                 \
+    if (err_ != hipSuccess)                                                                   \
+      return dgc_set_error(DGC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+#define DGC_REQUIRE(cond, msg) \
+  do {                         \
+    if (!(cond)) return dgc_set_error(DGC_ERR_INVALID, msg); \
+  } while (0)
+#define DGC_LAUNCH(call)                                                                        \
+  do {                                                                                          \
+    hipError_t err_ = (call);                                                                   \
+    if (err_ != hipSuccess)                                                                     \
+      return dgc_set_error(DGC_ERR_LAUNCH, std::string("failed to launch kernel of device function phase! ") + \
+                                               hipGetErrorString(err_));                        \
+  } while (0)
+
+namespace {
+
+template <typename T>
+int alloc(T** p, size_t n) {
+  DGC_HIP_TRY(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(T)));
+  return DGC_OK;
+}
+
+template <typename T>
+int upload(T** p, const std::vector<T>& h) {
+  int rc = alloc(p, h.size());
+  if (rc) return rc;
+  DGC_HIP_TRY(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DGC_OK;
+}
+
+// Transposed, row-padded operator layout read by the tile kernels.
+std::vector<double> transpose_pad(const double* A, int rows, int cols) {
+  const int Npad = dgc::padded_rows(rows);
+  std::vector<double> t(size_t(cols) * Npad, 0.0);
+  for (int j = 0; j < rows; ++j)
+    for (int k = 0; k < cols; ++k) t[size_t(k) * Npad + j] = A[size_t(j) * cols + k];
+  return t;
+}
+
+template <typename T>
+std::vector<T> cast(const std::vector<double>& v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+__global__ void axpby_kernel(double a, const double* x, double b, const double* y, double* out, size_t n) {
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = a * x[i] + b * y[i];
+}
+
+// out32 = float(s * x64)
+__global__ void to_float_kernel(const double* x, double s, float* out, size_t n) {
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = float(s * x[i]);
+}
+
+// x64 += s * double(e32)
+__global__ void add_scaled_float_kernel(double* x, double s, const float* e, size_t n) {
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) x[i] += s * double(e[i]);
+}
+
+// probe vector: 1 at node k of every cell of colour `col`, 0 elsewhere
+__global__ void probe_fill_kernel(const int* color, int col, int k, int Np, int K, double* u) {
+  const size_t n = size_t(Np) * K;
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int j = int(i / K), c = int(i % K);
+    u[i] = (j == k && color[c] == col) ? 1.0 : 0.0;
+  }
+}
+
+// blocks[(j*Np + k) * K + c] = out[j*K + c] for the cells of colour `col`
+__global__ void probe_extract_kernel(const int* color, int col, int k, int Np, int K, const double* out,
+                                     double* blocks) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= K || color[c] != col) return;
+  for (int j = 0; j < Np; ++j) blocks[(size_t(j) * Np + k) * K + c] = out[size_t(j) * K + c];
+}
+
+unsigned grid_for(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 8192)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+template <typename T>
+int IpdgLevel<T>::init(const dgc_ipdg_s& s, const std::vector<double>& drT, const std::vector<double>& dsT,
+                       const std::vector<double>& lT, const std::vector<double>& mT,
+                       const std::vector<double>& geo, const std::vector<double>& J,
+                       const std::vector<double>& tau) {
+  const size_t n = size_t(s.disc->K) * s.disc->Np;
+  const size_t nf = size_t(s.disc->K) * 3 * s.disc->Nfp;
+  int rc;
+  if ((rc = upload(&DrT, cast<T>(drT))) || (rc = upload(&DsT, cast<T>(dsT))) || (rc = upload(&LT, cast<T>(lT))) ||
+      (rc = upload(&MT, cast<T>(mT))) || (rc = upload(&tau_d, cast<T>(tau))))
+    return rc;
+  if (std::is_same<T, double>::value) {
+    geo_d = reinterpret_cast<T*>(s.disc->geo);
+    J_d = reinterpret_cast<T*>(s.disc->J);
+    own_geo = false;
+  } else {
+    if ((rc = upload(&geo_d, cast<T>(geo))) || (rc = upload(&J_d, cast<T>(J)))) return rc;
+    own_geo = true;
+  }
+  if ((rc = alloc(&qx, n)) || (rc = alloc(&qy, n)) || (rc = alloc(&du, nf)) || (rc = alloc(&r, n)) ||
+      (rc = alloc(&z, n)) || (rc = alloc(&p, n)) || (rc = alloc(&q, n)) || (rc = alloc(&x, n)) ||
+      (rc = alloc(&partial, 2 * size_t(s.grid))) || (rc = alloc(&scal, dgc::kCgScalars)))
+    return rc;
+  return DGC_OK;
+}
+
+template <typename T>
+void IpdgLevel<T>::release() {
+  (void)hipFree(DrT);
+  (void)hipFree(DsT);
+  (void)hipFree(LT);
+  (void)hipFree(MT);
+  (void)hipFree(tau_d);
+  if (own_geo) {
+    (void)hipFree(geo_d);
+    (void)hipFree(J_d);
+  }
+  (void)hipFree(qx);
+  (void)hipFree(qy);
+  (void)hipFree(du);
+  (void)hipFree(r);
+  (void)hipFree(z);
+  (void)hipFree(p);
+  (void)hipFree(q);
+  (void)hipFree(x);
+  (void)hipFree(partial);
+  (void)hipFree(scal);
+  (void)hipFree(P);
+  if (graph) (void)hipGraphExecDestroy(graph);
+  *this = IpdgLevel<T>();
+}
+
+template <typename T>
+dgc::IpdgParams<T> IpdgLevel<T>::params(const dgc_ipdg_s& s) const {
+  dgc::IpdgParams<T> pr{};
+  pr.K = s.disc->K;
+  pr.DrT = DrT;
+  pr.DsT = DsT;
+  pr.LT = LT;
+  pr.MT = MT;
+  pr.geo = geo_d;
+  pr.J = J_d;
+  pr.tau = tau_d;
+  pr.nbr = s.disc->nbr;
+  pr.qx = qx;
+  pr.qy = qy;
+  pr.du = du;
+  return pr;
+}
+
+template struct IpdgLevel<double>;
+template struct IpdgLevel<float>;
+
+// Enqueue CG initialisation: r = b - S x (x given), z = P r, p = z, scalars.
+template <typename T>
+static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_times_bb, int maxit,
+                   bool x_is_zero, double* history) {
+  hipStream_t st = s->disc->ctx->stream;
+  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  // done = 1 while initialising: cg_update then leaves x and r alone and only
+  // forms z = P r and the partial sums; the kInit reduction clears the flag.
+  T h[dgc::kCgScalars] = {0, 0, 0, 0, 0, 1, T(tol_rel2_times_bb), T(maxit)};
+  DGC_HIP_TRY(hipMemcpyAsync(L.scal, h, sizeof(h), hipMemcpyHostToDevice, st));
+  if (x_is_zero) {
+    DGC_HIP_TRY(hipMemcpyAsync(L.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+  } else {
+    dgc::IpdgParams<T> pr = L.params(*s);
+    pr.u = L.x;
+    pr.out = L.q;
+    DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpd, st));
+    // r = b - q  (kept in T; for T = double this is the FP64 residual)
+    if (std::is_same<T, double>::value) {
+      hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, (const double*)b, -1.0,
+                         (const double*)L.q, (double*)L.r, n);
+      DGC_LAUNCH(hipGetLastError());
+    } else {
+      return dgc_set_error(DGC_ERR_INVALID, "cg_init: non-zero initial guess needs FP64");
+    }
+  }
+  DGC_HIP_TRY(hipMemsetAsync(L.p, 0, n * sizeof(T), st));
+  // z = P r and partials with alpha = 0 (no x/r change)
+  dgc::CgParams<T> cp{};
+  cp.K = s->disc->K;
+  cp.x = L.x;
+  cp.r = L.r;
+  cp.p = L.p;
+  cp.q = L.q;
+  cp.z = L.z;
+  cp.P = L.use_P ? L.P : nullptr;
+  cp.scal = L.scal;
+  cp.partial = L.partial;
+  DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
+  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kInit, L.scal, history, st));
+  DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
+  return DGC_OK;
+}
+
+// One PCG iteration (5 kernels + the two passes of the operator).
+template <typename T>
+static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
+  hipStream_t st = s->disc->ctx->stream;
+  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  dgc::IpdgParams<T> pr = L.params(*s);
+  pr.u = L.p;
+  pr.out = L.q;
+  pr.dotv = L.p;
+  pr.partial = L.partial;
+  pr.scal = L.scal;
+  DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpd, st));
+  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterApply, L.scal, history, st));
+  dgc::CgParams<T> cp{};
+  cp.K = s->disc->K;
+  cp.x = L.x;
+  cp.r = L.r;
+  cp.p = L.p;
+  cp.q = L.q;
+  cp.z = L.z;
+  cp.P = L.use_P ? L.P : nullptr;
+  cp.scal = L.scal;
+  cp.partial = L.partial;
+  DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
+  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterUpdate, L.scal, history, st));
+  DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
+  return DGC_OK;
+}
+
+// Run iterations in graph-captured chunks until the device flags completion.
+template <typename T>
+static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, int* iters) {
+  hipStream_t st = s->disc->ctx->stream;
+  const int chunk = 16;
+  if (!L.graph || L.graph_hist != history || L.graph_P != L.use_P) {
+    if (L.graph) (void)hipGraphExecDestroy(L.graph);
+    L.graph = nullptr;
+    hipGraph_t g = nullptr;
+    DGC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = DGC_OK;
+    for (int i = 0; i < chunk && rc == DGC_OK; ++i) rc = cg_iteration(s, L, history);
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != DGC_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&L.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      L.graph = nullptr;
+      return dgc_set_error(DGC_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+    }
+    L.graph_hist = history;
+    L.graph_P = L.use_P;
+  }
+  T h[dgc::kCgScalars];
+  for (int done_chunks = 0;; ++done_chunks) {
+    DGC_HIP_TRY(hipMemcpyAsync(h, L.scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    if (h[dgc::kDone] != T(0)) break;
+    if (int(h[dgc::kIter]) >= maxit) break;
+    DGC_HIP_TRY(hipGraphLaunch(L.graph, st));
+  }
+  *iters = int(h[dgc::kIter]);
+  return DGC_OK;
+}
+
+static void face_penalty(const dgc_disc_s* d, int tau_mode, double tau_scale, std::vector<double>& tau) {
+  const int K = d->K, N = d->order;
+  tau.assign(3 * size_t(K), tau_scale);
+  if (tau_mode == 0) return;
+  for (int c = 0; c < K; ++c)
+    for (int e = 0; e < 3; ++e) {
+      double h = 2.0 * d->h_J[c] / d->h_face_J[3 * size_t(c) + e];
+      const int code = d->h_nbr[size_t(e) * K + c];
+      if ((code & 3) != dgc::kBoundaryCode) {
+        const int nc = code >> 2, nf = code & 3;
+        h = std::min(h, 2.0 * d->h_J[nc] / d->h_face_J[3 * size_t(nc) + nf]);
+      }
+      tau[size_t(e) * K + c] = tau_scale * (N + 1.0) * (N + 2.0) / h;
+    }
+}
+
+extern "C" {
+
+int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                    const double* mass, int tau_mode, double tau_scale, dgc_ipdg_t* out) {
+  DGC_REQUIRE(disc != nullptr && out != nullptr, "dgc_ipdg_create: null argument");
+  DGC_REQUIRE(Dr && Ds && lift && mass, "dgc_ipdg_create: null operator matrix");
+  DGC_REQUIRE(tau_mode == 0 || tau_mode == 1, "dgc_ipdg_create: tau_mode must be 0 or 1");
+  DGC_REQUIRE(std::isfinite(tau_scale) && tau_scale > 0, "dgc_ipdg_create: tau_scale must be positive");
+  DGC_HIP_TRY(hipSetDevice(disc->ctx->device));
+  auto* s = new dgc_ipdg_s;
+  s->disc = disc;
+  s->tau_mode = tau_mode;
+  s->tau_scale = tau_scale;
+  s->grid = (disc->K + dgc::kTile - 1) / dgc::kTile;
+  const int Np = disc->Np, NF = 3 * disc->Nfp, K = disc->K;
+  s->mass.assign(mass, mass + size_t(Np) * Np);
+  std::vector<double> tau;
+  face_penalty(disc, tau_mode, tau_scale, tau);
+  std::vector<double> geo(size_t(dgc::kGeoComps) * K), J(disc->h_J);
+  int rc = DGC_OK;
+  if (hipMemcpy(geo.data(), disc->geo, geo.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = dgc_set_error(DGC_ERR_HIP, "dgc_ipdg_create: geometry readback failed");
+  const auto drT = transpose_pad(Dr, Np, Np), dsT = transpose_pad(Ds, Np, Np);
+  const auto lT = transpose_pad(lift, Np, NF), mT = transpose_pad(mass, Np, Np);
+  if (rc == DGC_OK) rc = s->f64.init(*s, drT, dsT, lT, mT, geo, J, tau);
+  if (rc == DGC_OK) rc = s->f32.init(*s, drT, dsT, lT, mT, geo, J, tau);
+  if (rc == DGC_OK) rc = alloc(&s->hist, 1);
+  if (rc != DGC_OK) {
+    s->f64.release();
+    s->f32.release();
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return DGC_OK;
+}
+
+int dgc_ipdg_destroy(dgc_ipdg_t s) {
+  if (!s) return DGC_OK;
+  (void)hipStreamSynchronize(s->disc->ctx->stream);
+  s->f64.release();
+  s->f32.release();
+  (void)hipFree(s->hist);
+  (void)hipFree(s->color);
+  delete s;
+  return DGC_OK;
+}
+
+int dgc_ipdg_apply(dgc_ipdg_t s, double* u) {
+  DGC_REQUIRE(s && u, "dgc_ipdg_apply: null argument");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = u;
+  pr.out = u;  // pass 2 reads only q and du, so the in-place update is safe
+  DGC_LAUNCH(dgc::ipdg_apply<double>(s->disc->order, pr, dgc::kLaplace, s->disc->ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_ipdg_phase(dgc_ipdg_t s, int phase, double* u) {
+  DGC_REQUIRE(s && u, "dgc_ipdg_phase: null argument");
+  DGC_REQUIRE(phase == 1 || phase == 2, "dgc_ipdg_phase: phase must be 1 or 2");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = u;
+  pr.out = u;
+  DGC_LAUNCH(dgc::ipdg_apply<double>(s->disc->order, pr, phase == 1 ? dgc::kPhaseOne : dgc::kPhaseTwo,
+                                     s->disc->ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_ipdg_apply_spd(dgc_ipdg_t s, const double* u, double* out) {
+  DGC_REQUIRE(s && u && out && u != out, "dgc_ipdg_apply_spd: bad argument");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = u;
+  pr.out = out;
+  DGC_LAUNCH(dgc::ipdg_apply<double>(s->disc->order, pr, dgc::kSpd, s->disc->ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out) {
+  DGC_REQUIRE(s && f && out && f != out, "dgc_ipdg_mass: bad argument");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = f;
+  pr.out = out;
+  DGC_LAUNCH(dgc::ipdg_mass_apply<double>(s->disc->order, pr, s->disc->ctx->stream));
+  return DGC_OK;
+}
+
+int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors) {
+  DGC_REQUIRE(s != nullptr, "dgc_ipdg_block_jacobi: null solver");
+  dgc_disc_s* d = s->disc;
+  const int K = d->K, Np = d->Np;
+  hipStream_t st = d->ctx->stream;
+  // Distance-2 greedy colouring: (S u)_c depends on u in c and in cells up to
+  // two faces away, so cells of one colour can be probed together.
+  std::vector<int> nb(3 * size_t(K), -1);
+  for (int c = 0; c < K; ++c)
+    for (int e = 0; e < 3; ++e) {
+      const int code = d->h_nbr[size_t(e) * K + c];
+      if ((code & 3) != dgc::kBoundaryCode) nb[3 * size_t(c) + e] = code >> 2;
+    }
+  std::vector<int> color(K, -1), mark(64, -1);
+  int ncol = 0;
+  for (int c = 0; c < K; ++c) {
+    std::vector<int> near;
+    for (int e = 0; e < 3; ++e) {
+      const int a = nb[3 * size_t(c) + e];
+      if (a < 0) continue;
+      near.push_back(a);
+      for (int f = 0; f < 3; ++f) {
+        const int b = nb[3 * size_t(a) + f];
+        if (b >= 0 && b != c) near.push_back(b);
+      }
+    }
+    // also cells two faces away through either side (symmetric closure)
+    for (int a : near)
+      if (color[a] >= 0 && color[a] < 64) mark[color[a]] = c;
+    int col = 0;
+    while (col < 64 && mark[col] == c) ++col;
+    if (col >= 64) return dgc_set_error(DGC_ERR_UNSUPPORTED, "block-Jacobi colouring needs > 64 colours");
+    color[c] = col;
+    ncol = std::max(ncol, col + 1);
+  }
+  int rc;
+  if (!s->color && (rc = alloc(&s->color, K))) return rc;
+  DGC_HIP_TRY(hipMemcpy(s->color, color.data(), K * sizeof(int), hipMemcpyHostToDevice));
+  const size_t n = size_t(K) * Np;
+  double *u = s->f64.z, *o = s->f64.q, *blocks = nullptr;
+  if ((rc = alloc(&blocks, size_t(Np) * Np * K))) return rc;
+  for (int col = 0; col < ncol; ++col)
+    for (int k = 0; k < Np; ++k) {
+      hipLaunchKernelGGL(probe_fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, s->color, col, k, Np, K, u);
+      DGC_LAUNCH(hipGetLastError());
+      dgc::IpdgParams<double> pr = s->f64.params(*s);
+      pr.u = u;
+      pr.out = o;
+      DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
+      hipLaunchKernelGGL(probe_extract_kernel, dim3((K + 255) / 256), dim3(256), 0, st, s->color, col, k, Np, K,
+                         o, blocks);
+      DGC_LAUNCH(hipGetLastError());
+    }
+  std::vector<double> hb(size_t(Np) * Np * K), hinv(hb.size());
+  hipError_t e = hipMemcpyAsync(hb.data(), blocks, hb.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(blocks);
+  if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, "block readback failed");
+  dgc_host::Mat B(Np, Np);
+  for (int c = 0; c < K; ++c) {
+    for (int j = 0; j < Np; ++j)
+      for (int k = 0; k < Np; ++k) B(j, k) = hb[(size_t(j) * Np + k) * K + c];
+    dgc_host::Mat Bi;
+    try {
+      Bi = dgc_host::inverse(B);
+    } catch (const std::exception& ex) {
+      return dgc_set_error(DGC_ERR_INVALID, std::string("block-Jacobi: cell block not invertible: ") + ex.what());
+    }
+    for (int j = 0; j < Np; ++j)
+      for (int k = 0; k < Np; ++k) hinv[(size_t(j) * Np + k) * K + c] = Bi(j, k);
+  }
+  (void)hipFree(s->f64.P);
+  (void)hipFree(s->f32.P);
+  s->f64.P = nullptr;
+  s->f32.P = nullptr;
+  if ((rc = upload(&s->f64.P, hinv)) || (rc = upload(&s->f32.P, cast<float>(hinv)))) return rc;
+  s->diag_blocks = hb;
+  if (num_colors) *num_colors = ncol;
+  return DGC_OK;
+}
+
+int dgc_ipdg_diag_blocks(dgc_ipdg_t s, double* blocks) {
+  DGC_REQUIRE(s && blocks, "dgc_ipdg_diag_blocks: null argument");
+  DGC_REQUIRE(!s->diag_blocks.empty(), "dgc_ipdg_diag_blocks: call dgc_ipdg_block_jacobi first");
+  std::copy(s->diag_blocks.begin(), s->diag_blocks.end(), blocks);
+  return DGC_OK;
+}
+
+int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxit, int precond,
+                 int* iters, double* relres, double* history, float* ms) {
+  DGC_REQUIRE(s && b && x, "dgc_ipdg_pcg: null argument");
+  DGC_REQUIRE(rtol > 0 && maxit > 0, "dgc_ipdg_pcg: need rtol > 0 and maxit > 0");
+  DGC_REQUIRE(precond == 0 || precond == 1, "dgc_ipdg_pcg: precond must be 0 (none) or 1 (block-Jacobi)");
+  if (precond == 1 && !s->f64.P) {
+    int rc = dgc_ipdg_block_jacobi(s, nullptr);
+    if (rc) return rc;
+  }
+  dgc_disc_s* d = s->disc;
+  hipStream_t st = d->ctx->stream;
+  const size_t n = size_t(d->K) * d->Np;
+  IpdgLevel<double>& L = s->f64;
+  L.use_P = precond == 1;
+  if (s->hist_len < maxit + 1) {
+    (void)hipFree(s->hist);
+    s->hist = nullptr;
+    int rc = alloc(&s->hist, size_t(maxit) + 1);
+    if (rc) return rc;
+    s->hist_len = maxit + 1;
+    if (L.graph) (void)hipGraphExecDestroy(L.graph);
+    L.graph = nullptr;
+  }
+  // ||b||^2 on the host (set-up, outside the timed region)
+  std::vector<double> hb(n);
+  DGC_HIP_TRY(hipMemcpyAsync(hb.data(), b, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  double bb = 0.0;
+  for (double v : hb) bb += v * v;
+  DGC_REQUIRE(std::isfinite(bb), "dgc_ipdg_pcg: non-finite right-hand side");
+  if (bb == 0.0) {
+    DGC_HIP_TRY(hipMemsetAsync(x, 0, n * sizeof(double), st));
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    return DGC_OK;
+  }
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventRecord(e0, st));
+  DGC_HIP_TRY(hipMemsetAsync(L.x, 0, n * sizeof(double), st));
+  int total = 0, rc = DGC_OK;
+  double rel = 1.0;
+  // restart from the true residual if recurrence drift left it above rtol
+  for (int restart = 0; restart < 4 && rc == DGC_OK; ++restart) {
+    rc = cg_init<double>(s, L, b, rtol * rtol * bb, maxit - total, restart == 0, s->hist + total);
+    int it = 0;
+    if (rc == DGC_OK) rc = cg_run<double>(s, L, s->hist + total, maxit - total, &it);
+    total += it;
+    if (rc != DGC_OK) break;
+    // true residual r = b - S x
+    dgc::IpdgParams<double> pr = L.params(*s);
+    pr.u = L.x;
+    pr.out = L.q;
+    hipError_t le = dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st);
+    if (le != hipSuccess) {
+      rc = dgc_set_error(DGC_ERR_LAUNCH, "true residual apply failed");
+      break;
+    }
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)L.q, L.r, n);
+    std::vector<double> hr(n);
+    DGC_HIP_TRY(hipMemcpyAsync(hr.data(), L.r, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    double rr = 0.0;
+    for (double v : hr) rr += v * v;
+    rel = std::sqrt(rr / bb);
+    if (!(rel == rel)) {
+      rc = dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg: residual is not finite");
+      break;
+    }
+    if (rel <= rtol || total >= maxit) break;
+  }
+  DGC_HIP_TRY(hipEventRecord(e1, st));
+  DGC_HIP_TRY(hipEventSynchronize(e1));
+  float elapsed = 0.f;
+  (void)hipEventElapsedTime(&elapsed, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != DGC_OK) return rc;
+  DGC_HIP_TRY(hipMemcpyAsync(x, L.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (history) DGC_HIP_TRY(hipMemcpyAsync(history, s->hist, (total + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  if (iters) *iters = total;
+  if (relres) *relres = rel;
+  if (ms) *ms = elapsed;
+  if (rel > rtol) return dgc_set_error(DGC_ERR_NOT_CONVERGED, "dgc_ipdg_pcg: iteration cap reached");
+  return DGC_OK;
+}
+
+int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, double inner_rtol,
+                       int max_outer, int max_inner, int* outer_iters, int* inner_iters, double* relres,
+                       float* ms) {
+  DGC_REQUIRE(s && b && x, "dgc_ipdg_pcg_mixed: null argument");
+  DGC_REQUIRE(rtol > 0 && inner_rtol > 0 && inner_rtol < 1 && max_outer > 0 && max_inner > 0,
+              "dgc_ipdg_pcg_mixed: bad tolerance or iteration cap");
+  if (!s->f64.P) {
+    int rc = dgc_ipdg_block_jacobi(s, nullptr);
+    if (rc) return rc;
+  }
+  dgc_disc_s* d = s->disc;
+  hipStream_t st = d->ctx->stream;
+  const size_t n = size_t(d->K) * d->Np;
+  IpdgLevel<double>& H = s->f64;
+  IpdgLevel<float>& L = s->f32;
+  L.use_P = true;
+  std::vector<double> hv(n);
+  auto norm2 = [&](const double* v, double& out) -> int {
+    DGC_HIP_TRY(hipMemcpyAsync(hv.data(), v, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    double a = 0.0;
+    for (double t : hv) a += t * t;
+    out = a;
+    return DGC_OK;
+  };
+  double bb;
+  int rc = norm2(b, bb);
+  if (rc) return rc;
+  DGC_REQUIRE(std::isfinite(bb), "dgc_ipdg_pcg_mixed: non-finite right-hand side");
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventRecord(e0, st));
+  DGC_HIP_TRY(hipMemsetAsync(H.x, 0, n * sizeof(double), st));
+  DGC_HIP_TRY(hipMemcpyAsync(H.r, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
+  int outer = 0, inner_total = 0;
+  if (s->hist_len < max_inner + 1) {
+    (void)hipFree(s->hist);
+    s->hist = nullptr;
+    if ((rc = alloc(&s->hist, size_t(max_inner) + 1))) return rc;
+    s->hist_len = max_inner + 1;
+    if (L.graph) (void)hipGraphExecDestroy(L.graph);
+    L.graph = nullptr;
+    if (H.graph) (void)hipGraphExecDestroy(H.graph);
+    H.graph = nullptr;
+  }
+  while (rel > rtol && outer < max_outer) {
+    // inner FP32 solve of S e = r / ||r||
+    const double rn = std::sqrt(rr);
+    // The FP32 right-hand side goes to L.z: cg_init copies it into L.r before
+    // anything overwrites z.
+    hipLaunchKernelGGL(to_float_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const double*)H.r, 1.0 / rn, L.z, n);
+    DGC_LAUNCH(hipGetLastError());
+    DGC_HIP_TRY(hipMemsetAsync(L.x, 0, n * sizeof(float), st));
+    rc = cg_init<float>(s, L, L.z, inner_rtol * inner_rtol, max_inner, true, s->hist);
+    int it = 0;
+    if (rc == DGC_OK) rc = cg_run<float>(s, L, s->hist, max_inner, &it);
+    if (rc) return rc;
+    inner_total += it;
+    // x += ||r|| e;  r = b - S x (FP64)
+    hipLaunchKernelGGL(add_scaled_float_kernel, dim3(grid_for(n)), dim3(256), 0, st, H.x, rn, (const float*)L.x, n);
+    DGC_LAUNCH(hipGetLastError());
+    dgc::IpdgParams<double> pr = H.params(*s);
+    pr.u = H.x;
+    pr.out = H.q;
+    DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)H.q, H.r, n);
+    DGC_LAUNCH(hipGetLastError());
+    if ((rc = norm2(H.r, rr))) return rc;
+    rel = std::sqrt(rr / bb);
+    ++outer;
+    if (!(rel == rel)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg_mixed: residual is not finite");
+  }
+  DGC_HIP_TRY(hipEventRecord(e1, st));
+  DGC_HIP_TRY(hipEventSynchronize(e1));
+  float elapsed = 0.f;
+  (void)hipEventElapsedTime(&elapsed, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  DGC_HIP_TRY(hipMemcpyAsync(x, H.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  if (outer_iters) *outer_iters = outer;
+  if (inner_iters) *inner_iters = inner_total;
+  if (relres) *relres = rel;
+  if (ms) *ms = elapsed;
+  if (rel > rtol) return dgc_set_error(DGC_ERR_NOT_CONVERGED, "dgc_ipdg_pcg_mixed: outer iteration cap reached");
+  return DGC_OK;
+}
+
+}  // extern "C"

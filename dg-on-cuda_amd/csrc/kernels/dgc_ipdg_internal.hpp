@@ -1,0 +1,52 @@
+// Library-internal state of an IPDG solver handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "dg_internal.hpp"
+#include "dg_ipdg.hpp"
+
+struct dgc_ipdg_s;
+
+// Device state of one precision level (operators, geometry, CG vectors).
+template <typename T>
+struct IpdgLevel {
+  T* DrT = nullptr;
+  T* DsT = nullptr;
+  T* LT = nullptr;
+  T* MT = nullptr;
+  T* tau_d = nullptr;
+  T* geo_d = nullptr;  // FP64 level aliases the discretization's arrays
+  T* J_d = nullptr;
+  bool own_geo = false;
+  T *qx = nullptr, *qy = nullptr, *du = nullptr;
+  T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr;
+  T* partial = nullptr;
+  T* scal = nullptr;
+  T* P = nullptr;  // block-Jacobi inverse blocks [Np*Np][K]
+  bool use_P = false;
+  hipGraphExec_t graph = nullptr;
+  double* graph_hist = nullptr;
+  bool graph_P = false;
+
+  int init(const dgc_ipdg_s& s, const std::vector<double>& drT, const std::vector<double>& dsT,
+           const std::vector<double>& lT, const std::vector<double>& mT, const std::vector<double>& geo,
+           const std::vector<double>& J, const std::vector<double>& tau);
+  void release();
+  dgc::IpdgParams<T> params(const dgc_ipdg_s& s) const;
+};
+
+struct dgc_ipdg_s {
+  dgc_disc_s* disc = nullptr;
+  int tau_mode = 0;
+  double tau_scale = 1.0;
+  int grid = 0;
+  std::vector<double> mass;         // reference mass matrix (row-major)
+  std::vector<double> diag_blocks;  // S_cc blocks [Np*Np][K] once computed
+  int* color = nullptr;
+  double* hist = nullptr;  // residual history (squared norms)
+  int hist_len = 0;
+  IpdgLevel<double> f64;
+  IpdgLevel<float> f32;
+};

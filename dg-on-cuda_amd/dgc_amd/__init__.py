@@ -18,4 +18,4 @@ from .core import (  # noqa: F401
     node_to_cell_major,
     cell_to_node_major,
 )
-
+from .ipdg import IPDG, poisson_rhs  # noqa: F401

@@ -69,6 +69,19 @@ def _declare(L):
                         C.POINTER(_vp)])
     d("dgc_rk4_steps_timed", [_vp, C.POINTER(_vp), C.c_size_t, C.c_double, C.c_double, C.c_int,
                               C.POINTER(_vp), C.POINTER(C.c_float)])
+    d("dgc_rk4_profile_stages", [_vp, C.POINTER(_vp), C.c_size_t, C.c_double, C.c_double, C.c_int,
+                                 C.POINTER(_vp), C.POINTER(C.c_float)])
+    d("dgc_ipdg_create", [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.POINTER(_vp)])
+    d("dgc_ipdg_destroy", [_vp])
+    d("dgc_ipdg_apply", [_vp, _vp])
+    d("dgc_ipdg_apply_spd", [_vp, _vp, _vp])
+    d("dgc_ipdg_mass", [_vp, _vp, _vp])
+    d("dgc_ipdg_block_jacobi", [_vp, _ip])
+    d("dgc_ipdg_diag_blocks", [_vp, _dp])
+    d("dgc_ipdg_pcg", [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _ip, _dp, _dp,
+                       C.POINTER(C.c_float)])
+    d("dgc_ipdg_pcg_mixed", [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, _ip, _ip, _dp,
+                             C.POINTER(C.c_float)])
     for name, args in _EXTRA_DECLS:
         d(name, args)
 
@@ -372,6 +385,13 @@ class _Operator:
         check(lib().dgc_rk4_steps_timed(self._h, self._fp(state), self.disc.size, t0, dt, nsteps,
                                         _ptrs([w.ptr for w in work]), C.byref(ms)))
         return ms.value
+
+    def profile_stages(self, state, t0, dt, nsteps, work):
+        """Mean duration (ms) of each of the 4 stage kernels over nsteps steps."""
+        ms = (C.c_float * 4)()
+        check(lib().dgc_rk4_profile_stages(self._h, self._fp(state), self.disc.size, t0, dt, nsteps,
+                                           _ptrs([w.ptr for w in work]), ms))
+        return list(ms)
 
     def __del__(self):
         # a closed context has already released the device; skip the call

@@ -1,0 +1,107 @@
+"""IPDG Poisson operator and solvers (ctypes over dgc_ipdg_* in include/dgc_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .core import DeviceArray, Discretization, _d, _vp, check, lib, ref_operators
+
+
+class IPDG:
+    """laplace_2d on the GPU: Lap_h with homogeneous Dirichlet data.
+
+    tau_mode 0 is the reference's tau = 1; tau_mode 1 is (N+1)(N+2)/h per face.
+    """
+
+    def __init__(self, disc: Discretization, tau_mode: int = 0, tau_scale: float = 1.0,
+                 ops: dict | None = None):
+        self.disc = disc
+        self.ctx = disc.ctx
+        ops = ops if ops is not None else ref_operators(disc.order, "strong")
+        self.ops = ops
+        h = _vp()
+        check(lib().dgc_ipdg_create(disc.handle, _d(np.ascontiguousarray(ops["Dr"])),
+                                    _d(np.ascontiguousarray(ops["Ds"])),
+                                    _d(np.ascontiguousarray(ops["LIFT"])),
+                                    _d(np.ascontiguousarray(ops["M"])), tau_mode, tau_scale, C.byref(h)))
+        self._h = h
+        self.num_colors = None
+
+    @property
+    def size(self):
+        return self.disc.size
+
+    def apply(self, u: DeviceArray):
+        check(lib().dgc_ipdg_apply(self._h, _vp(u.ptr)))
+
+    def apply_spd(self, u: DeviceArray, out: DeviceArray):
+        check(lib().dgc_ipdg_apply_spd(self._h, _vp(u.ptr), _vp(out.ptr)))
+
+    def mass(self, f: DeviceArray, out: DeviceArray):
+        check(lib().dgc_ipdg_mass(self._h, _vp(f.ptr), _vp(out.ptr)))
+
+    def block_jacobi(self) -> int:
+        n = C.c_int()
+        check(lib().dgc_ipdg_block_jacobi(self._h, C.byref(n)))
+        self.num_colors = n.value
+        return n.value
+
+    def diag_blocks(self) -> np.ndarray:
+        """Diagonal blocks of -M Lap_h, shape (K, Np, Np)."""
+        K, Np = self.disc.K, self.disc.Np
+        b = np.zeros(Np * Np * K)
+        check(lib().dgc_ipdg_diag_blocks(self._h, _d(b)))
+        return b.reshape(Np, Np, K).transpose(2, 0, 1).copy()
+
+    def pcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-10, maxit=20000, precond=1,
+            history=False) -> dict:
+        it = C.c_int()
+        rel = C.c_double()
+        ms = C.c_float()
+        hist = np.zeros(maxit + 1) if history else None
+        rc = lib().dgc_ipdg_pcg(self._h, _vp(b.ptr), _vp(x.ptr), rtol, maxit, precond, C.byref(it),
+                                C.byref(rel), _d(hist) if history else None, C.byref(ms))
+        check(rc)
+        out = dict(iters=it.value, relres=rel.value, ms=ms.value)
+        if history:
+            out["history"] = np.sqrt(hist[: it.value + 1])
+        return out
+
+    def pcg_mixed(self, b: DeviceArray, x: DeviceArray, rtol=1e-10, inner_rtol=1e-4, max_outer=30,
+                  max_inner=20000) -> dict:
+        o = C.c_int()
+        i = C.c_int()
+        rel = C.c_double()
+        ms = C.c_float()
+        check(lib().dgc_ipdg_pcg_mixed(self._h, _vp(b.ptr), _vp(x.ptr), rtol, inner_rtol, max_outer,
+                                       max_inner, C.byref(o), C.byref(i), C.byref(rel), C.byref(ms)))
+        return dict(outer=o.value, inner_iters=i.value, relres=rel.value, ms=ms.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self.ctx._h is not None:
+            lib().dgc_ipdg_destroy(self._h)
+            self._h = None
+
+
+def poisson_rhs(x, y, seed=3, nmodes=16):
+    """Right-hand side of the Poisson benchmark problem on [-1,1]^2 and its exact solution.
+
+    f = 2 pi^2 sin(pi x) sin(pi y) + sum over (m, n) in {1..4}^2 of
+    a_mn sin(m pi (x+1)/2) sin(n pi (y+1)/2), a_mn ~ N(0,1) (m^2+n^2)^-1
+    (amplitude decaying as k^-2 with k = |(m, n)|). The modes are Dirichlet
+    eigenfunctions, so u = sin(pi x) sin(pi y) + sum a_mn / lambda_mn phi_mn.
+    """
+    rng = np.random.default_rng(seed)
+    side = int(round(np.sqrt(nmodes)))
+    f = 2 * np.pi**2 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    u = np.sin(np.pi * x) * np.sin(np.pi * y)
+    for m in range(1, side + 1):
+        for n in range(1, side + 1):
+            k2 = m * m + n * n
+            a = rng.standard_normal() / k2
+            phi = np.sin(m * np.pi * (x + 1) / 2) * np.sin(n * np.pi * (y + 1) / 2)
+            lam = (np.pi / 2) ** 2 * k2
+            f = f + a * phi
+            u = u + a / lam * phi
+    return f, u

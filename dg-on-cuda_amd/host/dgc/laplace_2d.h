@@ -1,0 +1,65 @@
+// IPDG Laplacian with the host API of src/laplace_2d.h:44-81 and the
+// Poisson problem of examples/poisson_2d/poisson_2d.h (homogeneous Dirichlet
+// data). operator() applies Lap_h in place on a device vector (node-major);
+// solve() replaces the unit-vector assembly + sparse direct solve of
+// examples/poisson_2d/main.cpp:132-223 by block-Jacobi PCG on -M Lap_h.
+#ifndef DGC_LAPLACE_2D_H
+#define DGC_LAPLACE_2D_H
+
+#include <memory>
+#include <vector>
+
+#include "kernel_adapter.h"
+#include "simple_discretization_2d.h"
+
+BEGIN_NAMESPACE
+
+template <typename T, typename M>
+class laplace_2d : public simple_discretization_2d<T, M> {
+ public:
+  // tau_mode 0: tau = 1 as the reference; 1: (N+1)(N+2)/h per face
+  laplace_2d(const M& mesh, int order, int tau_mode = 0, dgc_ctx_t ctx = default_context())
+      : simple_discretization_2d<T, M>(mesh, order), m_disc(create_on_device(*this, ctx)) {
+    const int Np = (order + 1) * (order + 2) / 2;
+    m_Dr.resize(Np * Np);
+    m_Ds.resize(Np * Np);
+    m_L.resize(Np * 3 * (order + 1));
+    m_M.resize(Np * Np);
+    check(dgc_ref_operators(order, 1, nullptr, nullptr, nullptr, m_Dr.data(), m_Ds.data(), m_L.data(), m_M.data()),
+          "reference operators failed");
+    dgc_ipdg_t h = nullptr;
+    check(dgc_ipdg_create(m_disc.get(), m_Dr.data(), m_Ds.data(), m_L.data(), m_M.data(), tau_mode, 1.0, &h),
+          "failed to create the IPDG device object");
+    m_op = std::make_unique<device_laplace_op>(h);
+  }
+
+  // in place: u <- Lap_h u (homogeneous Dirichlet data on every boundary face)
+  void operator()(T* d_u) const {
+    kernel_adapter<device_laplace_op> ka(m_op.get(), 0, 0, this->total_num_nodes());
+    ka.execute_phase(phase_one{}, d_u);
+    ka.execute_phase(phase_two{}, d_u);
+  }
+
+  // -Lap_h u = f: b = M f, then block-Jacobi PCG to the relative residual rtol.
+  // Returns the iteration count.
+  int solve(const T* d_f, T* d_u, double rtol = 1e-10, int maxit = 100000) const {
+    device_vector<T> b(this->total_num_nodes());
+    check(dgc_ipdg_mass(m_op->handle(), d_f, b.data()), "mass apply failed");
+    int it = 0;
+    double rel = 0.0;
+    float ms = 0.f;
+    check(dgc_ipdg_pcg(m_op->handle(), b.data(), d_u, rtol, maxit, 1, &it, &rel, nullptr, &ms), "PCG failed");
+    return it;
+  }
+
+  dgc_ipdg_t handle() const { return m_op->handle(); }
+
+ private:
+  device_discretization m_disc;
+  std::unique_ptr<device_laplace_op> m_op;
+  std::vector<T> m_Dr, m_Ds, m_L, m_M;
+};
+
+END_NAMESPACE
+
+#endif

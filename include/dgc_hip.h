@@ -143,11 +143,61 @@ int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, doubl
 int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, double dt, int nsteps,
                   double* const* wk);
 
+/* ---- IPDG Poisson operator and solvers -----------------------------------
+ * dgc_ipdg_create replaces the laplace_2d constructor (src/laplace_2d.h:83-119)
+ * and the device object of examples/poisson_2d/k_poisson_2d.cu:44-97. Dr, Ds
+ * [Np*Np] and lift [Np*3Nfp] are the strong-form matrices, mass [Np*Np] is
+ * (V V^T)^{-1}. tau_mode 0: the reference's tau = 1 (src/laplace_2d.h:227-245);
+ * tau_mode 1: tau = (N+1)(N+2)/h per face, h = min over both sides of
+ * 2J/face_J. tau_scale multiplies either. Every boundary face is homogeneous
+ * Dirichlet (examples/poisson_2d/poisson_2d.h:73-79). */
+typedef struct dgc_ipdg_s* dgc_ipdg_t;
+int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                    const double* mass, int tau_mode, double tau_scale, dgc_ipdg_t* out);
+int dgc_ipdg_destroy(dgc_ipdg_t s);
+/* In place u <- Lap_h u: laplace_2d::operator() with homogeneous Dirichlet
+ * data (src/laplace_2d.h:121-128); replaces op_on_device
+ * (examples/poisson_2d/k_poisson_2d.cu:99-108). u: K*Np node-major. */
+int dgc_ipdg_apply(dgc_ipdg_t s, double* u);
+/* One phase of the in-place apply, as launched by kernel_adapter::execute_phase
+ * (src/kernel_adapter.cuh:77-82; examples/poisson_2d/k_poisson_2d.cu:105-107):
+ * phase 1 computes q = grad u and the jumps into the solver's workspace,
+ * phase 2 overwrites u with div q minus the lifted penalty terms. */
+int dgc_ipdg_phase(dgc_ipdg_t s, int phase, double* u);
+/* out = -M Lap_h u, the symmetric positive definite form (M = J * mass per cell). */
+int dgc_ipdg_apply_spd(dgc_ipdg_t s, const double* u, double* out);
+/* out = M f (mass-weighted right-hand side of -Lap u = f). */
+int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out);
+/* Builds the block-Jacobi preconditioner: the exact diagonal blocks of
+ * -M Lap_h, probed with a distance-2 colouring of the cells, inverted per cell. */
+int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);
+/* Copies the diagonal blocks [Np*Np][K] (entry (j,k) of cell c at (j*Np+k)*K+c). */
+int dgc_ipdg_diag_blocks(dgc_ipdg_t s, double* blocks);
+/* Conjugate gradients on (-M Lap_h) x = b to ||b - S x|| <= rtol ||b|| (FP64
+ * true residual, restarted if the recurrence drifts). precond 0: none,
+ * 1: block-Jacobi. x: device output (initial guess 0). history (host, may be
+ * NULL): maxit+1 squared recurrence residuals. *ms: solve time (HIP events).
+ * Replaces the assembly + sparse direct solve of examples/poisson_2d/main.cpp:132-223. */
+int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxit, int precond,
+                 int* iters, double* relres, double* history, float* ms);
+/* Mixed precision: FP32 operator, vectors and block-Jacobi PCG to inner_rtol
+ * inside an FP64 iterative-refinement loop whose residual b - S x is
+ * recomputed in FP64 each outer step; stops at FP64 relative residual rtol. */
+int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, double inner_rtol,
+                       int max_outer, int max_inner, int* outer_iters, int* inner_iters,
+                       double* relres, float* ms);
+
 /* Same as dgc_rk4_steps, bracketed by HIP events on the context stream;
  * blocks until done and returns the elapsed milliseconds in *ms
  * (measurement entry point; no reference counterpart). */
 int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
                         int nsteps, double* const* wk, float* ms);
+
+/* Runs nsteps single steps with each of the 4 stage kernels bracketed by HIP
+ * events; stage_ms[4] receives the mean duration of each stage kernel
+ * (measurement entry point; no reference counterpart). */
+int dgc_rk4_profile_stages(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
+                           int nsteps, double* const* wk, float* stage_ms);
 
 #ifdef __cplusplus
 }

@@ -251,6 +251,32 @@ int main(int argc, char** argv) {
     return 0;
   }
 
+  if (cmd == "advection_time" && argc == 6) {
+    // Timing only: args mesh N nsteps dt; prints "steps seconds checksum" for
+    // the reference advection time loop (examples/advection_2d/main.cpp:135-152).
+    Mesh m = load_mesh(argv[2]);
+    const int N = std::atoi(argv[3]);
+    const int nsteps = std::atoi(argv[4]);
+    const double dt = std::atof(argv[5]);
+    advection_2d<double, Mesh> op(m, N);
+    const int n = op.total_num_nodes();
+    std::vector<dgc::point_2d<double>> x(n);
+    std::vector<double> u(n), w0(n), w1(n), w2(n), w3(n), w4(n);
+    op.initialize_dofs(x.begin(), u.begin());
+    double t = 0.0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nsteps; ++i) {
+      dgc::rk4(u.data(), n, t, dt, op, &dgc::axpy_n<double, const double*, double*>, w0.data(), w1.data(),
+               w2.data(), w3.data(), w4.data());
+      t += dt;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    double chk = 0.0;
+    for (double v : u) chk += v;
+    std::printf("%d %.9f %.17g\n", nsteps, std::chrono::duration<double>(t1 - t0).count(), chk);
+    return 0;
+  }
+
   if (cmd == "laplace" && argc == 6) {
     // poisson_2d::operator() (laplace_2d with homogeneous Dirichlet) applied to
     // each vector of an input file of nvec * K*Np f64 (CPU layout).

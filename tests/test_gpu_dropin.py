@@ -1,0 +1,39 @@
+"""The C++ drop-in layer (dg-on-cuda_amd/host/dgc/*.h) end to end: the example
+drivers written against it run on the GPU and reproduce the analytic
+solutions, as the reference's example programs do."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "dg-on-cuda_amd", "bin")
+
+
+def run(exe, *args):
+    out = subprocess.run([os.path.join(BIN, exe), *map(str, args)], capture_output=True, text=True, timeout=300)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_maxwell_driver_cavity_mode():
+    out = run("maxwell_2d", 4, "structured")
+    m = re.search(r"steps = (\d+).*error norm = ([0-9.e+-]+)", out)
+    assert m and int(m.group(1)) == 491
+    assert float(m.group(2)) < 1e-8
+
+
+def test_maxwell_driver_on_reference_gmsh_mesh():
+    out = run("maxwell_2d", 3, os.path.join(GOLDEN, "square_domain_coarse.msh"))
+    assert "error norm" in out
+
+
+@pytest.mark.parametrize("tau_mode", [0, 1])
+def test_poisson_driver(tau_mode):
+    out = run("poisson_2d", 3, "structured", tau_mode)
+    m = re.search(r"max error = ([0-9.e+-]+)", out)
+    assert m and float(m.group(1)) < 1e-3

@@ -1,0 +1,172 @@
+"""GPU parity of the IPDG operator and its solvers against the reference's
+golden vectors and the oracle (FP64: relative L2 <= 1e-12 for the operator;
+the solvers are judged by the oracle's FP64 true residual)."""
+import numpy as np
+import pytest
+from scipy.linalg import block_diag
+
+import dgc_amd
+from dgc_amd import cell_to_node_major as c2n, node_to_cell_major as n2c
+from oracle_lib import Problem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _from_fixture(fix, prefix=""):
+    return dgc_amd.Mesh.from_arrays(fix[prefix + "vx"], fix[prefix + "vy"], fix[prefix + "etov"])
+
+
+def test_apply_matches_reference(ctx, golden):
+    fix = golden("laplace_ref.npz")
+    N = int(fix["order"])
+    mesh = _from_fixture(fix)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=0)
+    K, Np = disc.K, disc.Np
+    for vin, vout in zip(fix["vin"], fix["vout"]):
+        u = dgc_amd.DeviceArray.from_host(ctx, c2n(vin, K, Np))
+        op.apply(u)
+        assert rel(n2c(u.download(), K, Np), vout) < 1e-13
+
+
+def test_dense_operator_matches_reference(ctx, golden):
+    fix = golden("laplace_ref.npz")
+    N = int(fix["coarse_order"])
+    mesh = _from_fixture(fix, "coarse_")
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=0)
+    K, Np = disc.K, disc.Np
+    n = K * Np
+    cols = []
+    for e in np.eye(n):
+        u = dgc_amd.DeviceArray.from_host(ctx, c2n(e, K, Np))
+        op.apply(u)
+        cols.append(n2c(u.download(), K, Np))
+    A = np.stack(cols, axis=1)
+    np.testing.assert_allclose(A, fix["coarse_A"], atol=1e-12 * np.abs(A).max(), rtol=0)
+
+
+@pytest.mark.parametrize("N,tau_mode", [(1, 1), (3, 1), (5, 0), (7, 1)])
+def test_spd_form_and_mass_vs_oracle(ctx, N, tau_mode):
+    mesh = dgc_amd.Mesh.structured(9, 8, -1, 1, -1, 1, jitter=0.1, seed=N)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=tau_mode)
+    K, Np = disc.K, disc.Np
+    P = Problem.from_mesh(mesh, N, form="strong")
+    v = np.random.default_rng(N).standard_normal(K * Np)
+    lap = P.laplace(v, tau_mode).reshape(K, Np)
+    M = dgc_amd.ref_operators(N, "strong")["M"]
+    J = disc.geometry["J"]
+    want = -(J[:, None] * (lap @ M.T)).reshape(-1)
+    u = dgc_amd.DeviceArray.from_host(ctx, c2n(v, K, Np))
+    out = dgc_amd.DeviceArray(ctx, K * Np)
+    op.apply_spd(u, out)
+    assert rel(n2c(out.download(), K, Np), want) < 1e-12
+    op.mass(u, out)
+    wm = (J[:, None] * (v.reshape(K, Np) @ M.T)).reshape(-1)
+    assert rel(n2c(out.download(), K, Np), wm) < 1e-13
+    # in-place Laplacian in the SIPG mode too
+    u2 = dgc_amd.DeviceArray.from_host(ctx, c2n(v, K, Np))
+    op.apply(u2)
+    assert rel(n2c(u2.download(), K, Np), lap.reshape(-1)) < 1e-12
+
+
+def _dense_spd(P, mesh, N, tau_mode):
+    n = P.K * P.Np
+    A = np.stack([P.laplace(e, tau_mode) for e in np.eye(n)], axis=1)
+    M = dgc_amd.ref_operators(N, "strong")["M"]
+    J = mesh.geometry(N)["J"]
+    return -block_diag(*[Jc * M for Jc in J]) @ A
+
+
+def test_block_jacobi_blocks_are_exact_diagonal_blocks(ctx):
+    N = 2
+    mesh = dgc_amd.Mesh.structured(5, 4, -1, 1, -1, 1, jitter=0.1, seed=2)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    ncol = op.block_jacobi()
+    assert 4 <= ncol <= 20
+    blocks = op.diag_blocks()
+    P = Problem.from_mesh(mesh, N, form="strong")
+    S = _dense_spd(P, mesh, N, 1)
+    Np = disc.Np
+    for c in range(disc.K):
+        want = S[c * Np:(c + 1) * Np, c * Np:(c + 1) * Np]
+        np.testing.assert_allclose(blocks[c], want, atol=1e-12 * np.abs(S).max(), rtol=0)
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_pcg_small_matches_direct_solve(ctx, precond):
+    N = 2
+    mesh = dgc_amd.Mesh.structured(6, 6, -1, 1, -1, 1, jitter=0.1, seed=5)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    K, Np = disc.K, disc.Np
+    P = Problem.from_mesh(mesh, N, form="strong")
+    S = _dense_spd(P, mesh, N, 1)
+    b = np.random.default_rng(3).standard_normal(K * Np)
+    xd = dgc_amd.DeviceArray(ctx, K * Np)
+    res = op.pcg(dgc_amd.DeviceArray.from_host(ctx, c2n(b, K, Np)), xd, rtol=1e-12, maxit=5000,
+                 precond=precond, history=True)
+    x = n2c(xd.download(), K, Np)
+    assert rel(S @ x, b) < 2e-12
+    assert rel(x, np.linalg.solve(S, b)) < 1e-9
+    assert res["relres"] <= 1e-12 and res["iters"] > 0
+    assert np.all(np.isfinite(res["history"]))
+
+
+def _poisson_problem(ctx, n_side, N, seed=3):
+    mesh = dgc_amd.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=seed)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    x, y = mesh.dof_coords(N, node_major=True)
+    f, u_exact = dgc_amd.poisson_rhs(x, y, seed=seed)
+    fd = dgc_amd.DeviceArray.from_host(ctx, f)
+    b = dgc_amd.DeviceArray(ctx, disc.size)
+    op.mass(fd, b)
+    return mesh, disc, op, b, u_exact
+
+
+def _oracle_relres(mesh, disc, x_node, b_node):
+    """FP64 true residual ||b - S x|| / ||b|| computed by the oracle."""
+    N, K, Np = disc.order, disc.K, disc.Np
+    P = Problem.from_mesh(mesh, N, form="strong")
+    x = n2c(x_node, K, Np)
+    lap = P.laplace(x, 1).reshape(K, Np)
+    M = dgc_amd.ref_operators(N, "strong")["M"]
+    Sx = -(disc.geometry["J"][:, None] * (lap @ M.T)).reshape(-1)
+    b = n2c(b_node, K, Np)
+    return np.linalg.norm(b - Sx) / np.linalg.norm(b)
+
+
+def test_poisson_32k_n3_block_jacobi_pcg_to_1e10(ctx):
+    """The Poisson benchmark problem: 128x128 squares (K = 32768), N = 3."""
+    mesh, disc, op, b, u_exact = _poisson_problem(ctx, 128, 3)
+    x = dgc_amd.DeviceArray(ctx, disc.size)
+    ncol = op.block_jacobi()
+    res = op.pcg(b, x, rtol=1e-10, maxit=20000, precond=1)
+    xs = x.download()
+    rr = _oracle_relres(mesh, disc, xs, b.download())
+    err = np.abs(xs - u_exact).max()
+    print(f"PCG(block-Jacobi) K={disc.K} N=3: {res['iters']} it, {res['ms']:.1f} ms, "
+          f"relres {res['relres']:.2e} (oracle {rr:.2e}), max err {err:.2e}, {ncol} colours")
+    assert res["relres"] <= 1e-10 and rr <= 2e-10
+    assert err < 1e-4
+
+
+def test_poisson_mixed_precision_refinement(ctx):
+    mesh, disc, op, b, u_exact = _poisson_problem(ctx, 64, 3)
+    x = dgc_amd.DeviceArray(ctx, disc.size)
+    res = op.pcg_mixed(b, x, rtol=1e-10, inner_rtol=1e-4)
+    xs = x.download()
+    rr = _oracle_relres(mesh, disc, xs, b.download())
+    x64 = dgc_amd.DeviceArray(ctx, disc.size)
+    r64 = op.pcg(b, x64, rtol=1e-10, precond=1)
+    print(f"mixed K={disc.K}: {res['outer']} outer, {res['inner_iters']} inner it, {res['ms']:.1f} ms; "
+          f"FP64 PCG {r64['iters']} it {r64['ms']:.1f} ms; oracle relres {rr:.2e}")
+    assert res["relres"] <= 1e-10 and rr <= 2e-10
+    assert rel(xs, x64.download()) < 1e-8
