@@ -64,7 +64,11 @@ struct StageParams {
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }
 
-hipError_t launch_stage(int order, int physics, int stage, const StageParams& p, hipStream_t s);
+// wt: waves per 64-cell tile (1, 2 or 4); a variant whose LDS would not fit
+// falls back to 4 (supported_waves_per_tile tells which one runs).
+hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s);
+int supported_waves_per_tile(int order, int physics, int wt);
+int default_waves_per_tile(int order, int physics);
 
 }  // namespace dgc
 

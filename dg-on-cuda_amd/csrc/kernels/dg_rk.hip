@@ -6,118 +6,193 @@
 //   src/device_SemiDiscOp_wrapper.cuh:35-58        (grid-stride launcher)
 //   src/explicit_runge_kutta.h:33-51 + src/axpy.h  (6 axpy passes per step, fused here)
 //
-// Mapping: one workgroup = 4 waves = a tile of 64 consecutive cells (lane =
-// cell). The tile's node rows are staged in LDS with coalesced 512-byte row
-// loads; the face traces of neighbours that live in the same tile are read
-// from LDS, others from global memory (L2). The face fluxes are computed
-// once per (cell, face node) and kept in LDS. Each wave then owns a block of
-// ceil(Np/4) output rows: the Dr/Ds/LIFT coefficients of those rows are
-// wave-uniform, so they are fetched with scalar loads into SGPRs and every
-// LDS read of a DOF feeds 2*rows FMAs. The RK4 stage update is applied
-// while the result is in registers, so each stage moves the state once.
+// Mapping. A tile is 64 consecutive cells (lane = cell) served by WT waves
+// (WT = 1, 2 or 4, chosen per order by the autotuner); a 256-thread
+// workgroup holds 4/WT tiles, and workgroups are renumbered so that each XCD
+// (own L2) sweeps a contiguous range of tiles, keeping neighbour traces in
+// the L2 that loaded them.
+//
+// Memory. Everything a tile needs from HBM is requested before the first
+// barrier: the connectivity codes, the node rows of the stage input, the four
+// metric terms (rx, ry, sx, sy) and the neighbour traces that live outside
+// the tile. Face normals and face_J/J are derived from the metric terms
+// (straight-sided cells), so a stage reads 4 doubles + 3 ints of geometry per
+// cell instead of 13 doubles. Node rows go to LDS; neighbours inside the tile
+// are read from LDS.
+//
+// Compute. Each wave owns ceil(Np/WT) output rows: their Dr/Ds/LIFT
+// coefficients are wave-uniform, so they arrive through scalar loads, and
+// each LDS read of a DOF feeds 2*rows FMAs. The RK4 stage update is applied
+// while the result is in registers (operand order of the reference's axpy).
 #include "dg_internal.hpp"
 
 namespace dgc {
 
-template <int N, int PHYS, int STAGE>
+// Bijective workgroup renumbering: consecutive hardware dispatch slots go to
+// the 8 XCDs in turn; give XCD x the contiguous range of logical groups
+// [x*per + min(x, rem), ...).
+__device__ __forceinline__ int xcd_group(int b, int G) {
+  constexpr int X = 8;
+  const int x = b % X, i = b / X, per = G / X, rem = G % X;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+template <int N, int PHYS, int STAGE, int WT>
 __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
   using T = Tri<N>;
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
   constexpr int F = PHYS == kMaxwellTM ? 3 : 1;
-  constexpr int RPW = (Np + kWaves - 1) / kWaves;
-  constexpr int Npad = RPW * kWaves;
-  __shared__ double us[F][Np][kTile];
-  __shared__ double fl[F][NF][kTile];
+  constexpr int TPW = kWaves / WT;        // tiles per workgroup
+  constexpr int RPW = (Np + WT - 1) / WT;  // output rows per wave (contiguous block)
+  constexpr int LPW = (Np + WT - 1) / WT;  // input rows loaded per wave (strided)
+  constexpr int FPW = (NF + WT - 1) / WT;  // face slots per wave (strided)
+  constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
+  __shared__ double us[TPW][F][Np][kTile];
+  __shared__ double fl[TPW][F][NF][kTile];
 
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
   const int wave = uniform(threadIdx.x >> 6);
-  const int c0 = blockIdx.x * kTile;
+  const int tw = wave / WT;
+  const int wt = wave - tw * WT;
+  const int c0 = (xcd_group(blockIdx.x, gridDim.x) * TPW + tw) * kTile;
   const int c = c0 + lane;
   const bool valid = c < K;
 
   double t = p.t;
   if (p.tclock != nullptr) t = p.tclock[0] + p.tfrac * p.dt_clock;
 
-  // 1. stage the tile's node rows in LDS
-  for (int j = wave; j < Np; j += kWaves) {
+  // ---- 1. issue every independent HBM read of the tile --------------------
+  int code[FPW];
 #pragma unroll
-    for (int f = 0; f < F; ++f) us[f][j][lane] = valid ? p.in[f][(size_t)j * K + c] : 0.0;
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wt + WT * i;
+    code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
+  }
+  double tl[F][LPW];
+#pragma unroll
+  for (int i = 0; i < LPW; ++i) {
+    const int j = wt + WT * i;
+#pragma unroll
+    for (int f = 0; f < F; ++f) tl[f][i] = (valid && j < Np) ? p.in[f][(size_t)j * K + c] : 0.0;
+  }
+  double rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0;
+  if (valid) {
+    rx = p.geo[(size_t)kRx * K + c];
+    ry = p.geo[(size_t)kRy * K + c];
+    sx = p.geo[(size_t)kSx * K + c];
+    sy = p.geo[(size_t)kSy * K + c];
+  }
+  double gv[F][FPW];  // neighbour traces outside the tile
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wt + WT * i;
+    const int e = m / Nfp, d = m - e * Nfp;
+    const int nf = code[i] & 3, idx = code[i] >> 2;
+    const bool far = nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile;
+    const int nn = T::fnode(nf & 3, Nfp - 1 - d);
+    (void)e;
+#pragma unroll
+    for (int f = 0; f < F; ++f) gv[f][i] = far ? p.in[f][(size_t)nn * K + idx] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < LPW; ++i) {
+    const int j = wt + WT * i;
+    if (j < Np) {
+#pragma unroll
+      for (int f = 0; f < F; ++f) us[tw][f][j][lane] = tl[f][i];
+    }
   }
   __syncthreads();
 
-  // 2. numerical fluxes at the face nodes, pre-scaled by face_J / J
-  for (int m = wave; m < NF; m += kWaves) {
-    const int e = m / Nfp;
-    const int d = m - e * Nfp;
+  // ---- 2. numerical fluxes at the face nodes, scaled by face_J / J ---------
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wt + WT * i;
+    if (m >= NF) break;
+    const int e = m / Nfp, d = m - e * Nfp;
     const int own = T::fnode(e, d);
     double a[F], b[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) a[f] = us[f][own][lane];
-    double nx = 0.0, ny = 0.0, fs = 0.0;
-    bool boundary = false;
-    if (valid) {
-      nx = p.geo[(size_t)(kNx0 + e) * K + c];
-      ny = p.geo[(size_t)(kNy0 + e) * K + c];
-      fs = p.geo[(size_t)(kFs0 + e) * K + c];
-      const int code = p.nbr[(size_t)e * K + c];
-      const int nf = code & 3;
-      const int idx = code >> 2;
-      if (nf == kBoundaryCode) {
-        boundary = true;
-        if (PHYS == kMaxwellTM) {
-          // PEC wall: (Hx, Hy, Ez)+ = (Hx, Hy, -Ez)-
-          b[0] = a[0];
-          if (F == 3) {
-            b[1] = a[1];
-            b[2] = -a[2];
-          }
-        } else {
-          if (p.bc == kBcSinSin) {
-            const double xb = p.bx[idx + d], yb = p.by[idx + d];
-            b[0] = sin((xb + 1.0 - t) * M_PI) * sin((yb + 1.0 - t) * M_PI);
-          } else {
-            b[0] = 0.0;
-          }
+    for (int f = 0; f < F; ++f) a[f] = us[tw][f][own][lane];
+    // outward normal and face_J/J of face e from the metric terms
+    const double vx = e == 0 ? sx : (e == 1 ? rx + sx : rx);
+    const double vy = e == 0 ? sy : (e == 1 ? ry + sy : ry);
+    const double fs = sqrt(vx * vx + vy * vy);
+    const double sg = (e == 1 ? 1.0 : -1.0) / (valid ? fs : 1.0);
+    const double nx = sg * vx, ny = sg * vy;
+    const int nf = code[i] & 3, idx = code[i] >> 2;
+    if (nf == kBoundaryCode) {
+      if (PHYS == kMaxwellTM) {
+        // PEC wall: (Hx, Hy, Ez)+ = (Hx, Hy, -Ez)-
+        b[0] = a[0];
+        if (F == 3) {
+          b[1 % F] = a[1 % F];
+          b[2 % F] = -a[2 % F];
         }
+      } else if (p.bc == kBcSinSin && valid) {
+        const double xb = p.bx[idx + d], yb = p.by[idx + d];
+        b[0] = sin((xb + 1.0 - t) * M_PI) * sin((yb + 1.0 - t) * M_PI);
       } else {
-        // neighbour traces run against this face's orientation
-        const int nn = T::fnode(nf, Nfp - 1 - d);
-        const int l = idx - c0;
-        if ((unsigned)l < (unsigned)kTile) {
-#pragma unroll
-          for (int f = 0; f < F; ++f) b[f] = us[f][nn][l];
-        } else {
-#pragma unroll
-          for (int f = 0; f < F; ++f) b[f] = p.in[f][(size_t)nn * K + idx];
-        }
+        b[0] = 0.0;
       }
     } else {
+      const int nn = T::fnode(nf, Nfp - 1 - d);
+      const int l = idx - c0;
+      if ((unsigned)l < (unsigned)kTile) {
 #pragma unroll
-      for (int f = 0; f < F; ++f) b[f] = a[f];
+        for (int f = 0; f < F; ++f) b[f] = us[tw][f][nn][l];
+      } else {
+#pragma unroll
+        for (int f = 0; f < F; ++f) b[f] = gv[f][i];
+      }
     }
-    (void)boundary;
     if (PHYS == kMaxwellTM) {
       const double dHx = a[0] - b[0];
       const double dHy = a[1 % F] - b[1 % F];
       const double sEz = a[2 % F] + b[2 % F];
       const double w = sEz + ny * dHx - nx * dHy;
-      fl[0][m][lane] = 0.5 * ny * w * fs;
+      fl[tw][0][m][lane] = 0.5 * ny * w * fs;
       if (F == 3) {
-        fl[1 % F][m][lane] = -0.5 * nx * w * fs;
-        fl[2 % F][m][lane] =
+        fl[tw][1 % F][m][lane] = -0.5 * nx * w * fs;
+        fl[tw][2 % F][m][lane] =
             0.5 * (ny * (a[0] + b[0]) - nx * (a[1 % F] + b[1 % F]) + a[2 % F] - b[2 % F]) * fs;
       }
     } else {
       const double an = p.ax * nx + p.ay * ny;
       const double U = an >= 0.0 ? a[0] : b[0];
-      fl[0][m][lane] = U * an * fs;
+      fl[tw][0][m][lane] = U * an * fs;
     }
   }
   __syncthreads();
 
-  // 3. volume term for this wave's rows: Dr.u and Ds.u per field
-  const int j0 = wave * RPW;
+  // ---- 3. stage operands of this wave's rows (in flight during the FMAs) --
+  const int j0 = wt * RPW;
+  double o1[F][RPW], o2[F][RPW], o3[F][RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = j0 + i;
+    const bool ok = valid && j < Np;
+    const size_t idx = (size_t)j * K + c;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      o1[f][i] = o2[f][i] = o3[f][i] = 0.0;
+      if (ok) {
+        if (STAGE == 2) {
+          o1[f][i] = p.acc[f][idx];
+          o2[f][i] = p.u[f][idx];
+        } else if (STAGE == 3) {
+          o2[f][i] = p.u[f][idx];
+        } else if (STAGE == 4) {
+          o1[f][i] = p.acc[f][idx];
+          o2[f][i] = p.u[f][idx];
+          o3[f][i] = p.k3[f][idx];
+        }
+      }
+    }
+  }
+
+  // ---- 4. volume term: Dr.u and Ds.u for this wave's rows -------------------
   double rr[F][RPW], ss[F][RPW];
 #pragma unroll
   for (int f = 0; f < F; ++f)
@@ -127,9 +202,9 @@ __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
   for (int k = 0; k < Np; ++k) {
     double uk[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) uk[f] = us[f][k][lane];
-    const double* dr = p.DrT + k * Npad + j0;
-    const double* ds = p.DsT + k * Npad + j0;
+    for (int f = 0; f < F; ++f) uk[f] = us[tw][f][k][lane];
+    const double* dr = p.DrT + k * S + j0;
+    const double* ds = p.DsT + k * S + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const double cr = dr[i], cs = ds[i];
@@ -140,38 +215,28 @@ __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
       }
     }
   }
-
   double res[F][RPW];
-  {
-    double rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0;
-    if (valid) {
-      rx = p.geo[(size_t)kRx * K + c];
-      ry = p.geo[(size_t)kRy * K + c];
-      sx = p.geo[(size_t)kSx * K + c];
-      sy = p.geo[(size_t)kSy * K + c];
-    }
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      if (PHYS == kMaxwellTM) {
-        // Hx_t = Dy Ez, Hy_t = -Dx Ez, Ez_t = Dy Hx - Dx Hy
-        res[0][i] = ry * rr[2 % F][i] + sy * ss[2 % F][i];
-        if (F == 3) {
-          res[1 % F][i] = -(rx * rr[2 % F][i] + sx * ss[2 % F][i]);
-          res[2 % F][i] = (ry * rr[0][i] + sy * ss[0][i]) - (rx * rr[1 % F][i] + sx * ss[1 % F][i]);
-        }
-      } else {
-        res[0][i] = (p.ax * rx + p.ay * ry) * rr[0][i] + (p.ax * sx + p.ay * sy) * ss[0][i];
+  for (int i = 0; i < RPW; ++i) {
+    if (PHYS == kMaxwellTM) {
+      // Hx_t = Dy Ez, Hy_t = -Dx Ez, Ez_t = Dy Hx - Dx Hy
+      res[0][i] = ry * rr[2 % F][i] + sy * ss[2 % F][i];
+      if (F == 3) {
+        res[1 % F][i] = -(rx * rr[2 % F][i] + sx * ss[2 % F][i]);
+        res[2 % F][i] = (ry * rr[0][i] + sy * ss[0][i]) - (rx * rr[1 % F][i] + sx * ss[1 % F][i]);
       }
+    } else {
+      res[0][i] = (p.ax * rx + p.ay * ry) * rr[0][i] + (p.ax * sx + p.ay * sy) * ss[0][i];
     }
   }
 
-  // 4. lift: res -= LIFT . (flux * face_J / J)
+  // ---- 5. lift: res -= LIFT . (flux * face_J / J) ----------------------------
 #pragma unroll
   for (int m = 0; m < NF; ++m) {
     double fm[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) fm[f] = fl[f][m][lane];
-    const double* lt = p.LT + m * Npad + j0;
+    for (int f = 0; f < F; ++f) fm[f] = fl[tw][f][m][lane];
+    const double* lt = p.LT + m * S + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const double cl = lt[i];
@@ -180,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
     }
   }
 
-  // 5. stage update (reference operand order: out = a * x + y)
+  // ---- 6. stage update (reference operand order: out = a * x + y) ------------
   if (!valid) return;
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
@@ -194,16 +259,16 @@ __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
         p.out[f][idx] = r;
       } else if (STAGE == 1) {
         p.acc[f][idx] = r;
-        p.out[f][idx] = p.c1 * r + us[f][j][lane];
+        p.out[f][idx] = p.c1 * r + us[tw][f][j][lane];
       } else if (STAGE == 2) {
-        p.acc[f][idx] = 2.0 * r + p.acc[f][idx];
-        p.out[f][idx] = p.c1 * r + p.u[f][idx];
+        p.acc[f][idx] = 2.0 * r + o1[f][i];
+        p.out[f][idx] = p.c1 * r + o2[f][i];
       } else if (STAGE == 3) {
         p.k3[f][idx] = r;
-        p.out[f][idx] = p.c1 * r + p.u[f][idx];
+        p.out[f][idx] = p.c1 * r + o2[f][i];
       } else {
-        const double w1 = 2.0 * p.k3[f][idx] + r;
-        const double w2 = p.c1 * p.acc[f][idx] + p.u[f][idx];
+        const double w1 = 2.0 * o3[f][i] + r;
+        const double w2 = p.c1 * o1[f][i] + o2[f][i];
         p.u[f][idx] = p.c1 * w1 + w2;
       }
     }
@@ -214,36 +279,88 @@ namespace {
 
 using StageFn = void (*)(const StageParams);
 
-template <int N, int PHYS>
+template <int N, int PHYS, int WT>
 constexpr StageFn stage_fn(int stage) {
-  return stage == 0 ? dg_stage_kernel<N, PHYS, 0>
-       : stage == 1 ? dg_stage_kernel<N, PHYS, 1>
-       : stage == 2 ? dg_stage_kernel<N, PHYS, 2>
-       : stage == 3 ? dg_stage_kernel<N, PHYS, 3>
-                    : dg_stage_kernel<N, PHYS, 4>;
+  return stage == 0 ? dg_stage_kernel<N, PHYS, 0, WT>
+       : stage == 1 ? dg_stage_kernel<N, PHYS, 1, WT>
+       : stage == 2 ? dg_stage_kernel<N, PHYS, 2, WT>
+       : stage == 3 ? dg_stage_kernel<N, PHYS, 3, WT>
+                    : dg_stage_kernel<N, PHYS, 4, WT>;
+}
+
+// A variant is built only if its LDS (all tiles of the workgroup) stays
+// within 80 KB, so that at least two workgroups share a CU; larger requests
+// fall back to one tile per workgroup (WT = 4).
+template <int N, int PHYS, int WT>
+constexpr bool fits() {
+  constexpr int Np = Tri<N>::Np, NF = 3 * Tri<N>::Nfp, F = PHYS == kMaxwellTM ? 3 : 1;
+  return WT == 4 || (kWaves / WT) * F * (Np + NF) * kTile * 8 <= 80 * 1024;
+}
+
+template <int N, int PHYS>
+StageFn pick_wt(int wt, int stage) {
+  if constexpr (fits<N, PHYS, 1>()) {
+    if (wt == 1) return stage_fn<N, PHYS, 1>(stage);
+  }
+  if constexpr (fits<N, PHYS, 2>()) {
+    if (wt == 2) return stage_fn<N, PHYS, 2>(stage);
+  }
+  return stage_fn<N, PHYS, 4>(stage);
+}
+
+template <int N, int PHYS>
+int effective_wt(int wt) {
+  if (wt == 1 && fits<N, PHYS, 1>()) return 1;
+  if (wt == 2 && fits<N, PHYS, 2>()) return 2;
+  return 4;
 }
 
 template <int PHYS>
-StageFn pick(int order, int stage) {
+StageFn pick(int order, int wt, int stage) {
   switch (order) {
-    case 1: return stage_fn<1, PHYS>(stage);
-    case 2: return stage_fn<2, PHYS>(stage);
-    case 3: return stage_fn<3, PHYS>(stage);
-    case 4: return stage_fn<4, PHYS>(stage);
-    case 5: return stage_fn<5, PHYS>(stage);
-    case 6: return stage_fn<6, PHYS>(stage);
-    case 7: return stage_fn<7, PHYS>(stage);
+    case 1: return pick_wt<1, PHYS>(wt, stage);
+    case 2: return pick_wt<2, PHYS>(wt, stage);
+    case 3: return pick_wt<3, PHYS>(wt, stage);
+    case 4: return pick_wt<4, PHYS>(wt, stage);
+    case 5: return pick_wt<5, PHYS>(wt, stage);
+    case 6: return pick_wt<6, PHYS>(wt, stage);
+    case 7: return pick_wt<7, PHYS>(wt, stage);
     default: return nullptr;
   }
 }
 
 }  // namespace
 
-hipError_t launch_stage(int order, int physics, int stage, const StageParams& p, hipStream_t s) {
-  if (stage < 0 || stage > 4) return hipErrorInvalidValue;
-  StageFn fn = physics == kMaxwellTM ? pick<kMaxwellTM>(order, stage) : pick<kAdvection>(order, stage);
+// The waves-per-tile value that will actually run for a request.
+int supported_waves_per_tile(int order, int physics, int wt) {
+#define DGC_WT_CASE(n) \
+  case n: return physics == kMaxwellTM ? effective_wt<n, kMaxwellTM>(wt) : effective_wt<n, kAdvection>(wt);
+  switch (order) {
+    DGC_WT_CASE(1)
+    DGC_WT_CASE(2)
+    DGC_WT_CASE(3)
+    DGC_WT_CASE(4)
+    DGC_WT_CASE(5)
+    DGC_WT_CASE(6)
+    DGC_WT_CASE(7)
+    default: return 4;
+  }
+#undef DGC_WT_CASE
+}
+
+int default_waves_per_tile(int order, int physics) {
+  const int want = physics == kAdvection ? (order <= 1 ? 1 : (order <= 3 ? 2 : 4)) : (order <= 1 ? 2 : 4);
+  return supported_waves_per_tile(order, physics, want);
+}
+
+hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s) {
+  if (stage < 0 || stage > 4 || (wt != 1 && wt != 2 && wt != 4)) return hipErrorInvalidValue;
+  wt = supported_waves_per_tile(order, physics, wt);
+  StageFn fn = physics == kMaxwellTM ? pick<kMaxwellTM>(order, wt, stage) : pick<kAdvection>(order, wt, stage);
   if (fn == nullptr) return hipErrorInvalidValue;
-  const int grid = (p.K + kTile - 1) / kTile;
+  const int tiles = (p.K + kTile - 1) / kTile;
+  const int tpw = kWaves / wt;
+  const int grid = (tiles + tpw - 1) / tpw;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }

@@ -41,6 +41,7 @@ struct dgc_op_s {
   double* LT = nullptr;
   double ax = 0.0, ay = 0.0;
   int bc = 0;
+  int wt = 4;  // waves per 64-cell tile of the stage kernels
   double* tclock = nullptr;
   // cached graph of one full RK4 step
   hipGraphExec_t graph = nullptr;
@@ -288,6 +289,7 @@ static int create_op(dgc_disc_t disc, int physics, const double* Dr, const doubl
   op->ax = ax;
   op->ay = ay;
   op->bc = bc;
+  op->wt = dgc::default_waves_per_tile(disc->order, physics);
   int rc = upload_operators(op, Dr, Ds, lift);
   if (rc != DGC_OK) {
     free_op(op);
@@ -356,7 +358,7 @@ int dgc_op_rhs(dgc_op_t op, const double* const* in, size_t size, double t, doub
     p.in[f] = in[f];
     p.out[f] = out[f];
   }
-  hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 0, p, op->disc->ctx->stream);
+  hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 0, op->wt, p, op->disc->ctx->stream);
   if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
   return DGC_OK;
 }
@@ -411,7 +413,7 @@ static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, dou
     }
     hipStream_t st = op->disc->ctx->stream;
     if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1)], st));
-    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, p, st);
+    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, op->wt, p, st);
     if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
     if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1) + 1], st));
   }
@@ -526,6 +528,74 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
   (void)hipEventDestroy(e1);
   if (err != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("timed replay: ") + hipGetErrorString(err));
   return DGC_OK;
+}
+
+int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
+  DGC_REQUIRE(op != nullptr, "dgc_op_set_waves_per_tile: null operator");
+  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4, "dgc_op_set_waves_per_tile: wt must be 1, 2 or 4");
+  op->wt = dgc::supported_waves_per_tile(op->disc->order, op->physics, wt);
+  if (op->graph) {
+    (void)hipGraphExecDestroy(op->graph);
+    op->graph = nullptr;
+  }
+  return DGC_OK;
+}
+
+int dgc_op_waves_per_tile(dgc_op_t op) { return op ? op->wt : 0; }
+
+// Times the stage-2 kernel (the widest: 5 state passes) of every supported
+// tiling on the caller's buffers and keeps the fastest. Only work vectors are
+// written (stage 2 writes wk0 and wk2, which the next step's stage 1
+// re-initialises), so the state is left untouched.
+int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
+                    float* ms_out /* [3]: wt = 1, 2, 4; -1 if not built */) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_op_autotune");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(reps > 0, "dgc_op_autotune: reps must be positive");
+  const int F = op->F;
+  dgc::StageParams p = base_params(op);
+  for (int f = 0; f < F; ++f) {
+    p.u[f] = inout[f];
+    p.acc[f] = wk[0 * F + f];
+    p.k3[f] = wk[3 * F + f];
+    p.in[f] = wk[1 * F + f];
+    p.out[f] = wk[2 * F + f];
+  }
+  p.c1 = 0.0;
+  hipStream_t st = op->disc->ctx->stream;
+  const int cand[3] = {1, 2, 4};
+  float best = 0.f;
+  int best_wt = op->wt;
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  for (int i = 0; i < 3; ++i) {
+    const int wt = cand[i];
+    if (ms_out) ms_out[i] = -1.f;
+    if (dgc::supported_waves_per_tile(op->disc->order, op->physics, wt) != wt) continue;
+    // warm-up launch, then timed repetitions
+    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 2, wt, p, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = dgc::launch_stage(op->disc->order, op->physics, 2, wt, p, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return dgc_set_error(DGC_ERR_LAUNCH, std::string("dgc_op_autotune: ") + hipGetErrorString(e));
+    }
+    ms /= reps;
+    if (ms_out) ms_out[i] = ms;
+    if (best == 0.f || ms < best) {
+      best = ms;
+      best_wt = wt;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return dgc_op_set_waves_per_tile(op, best_wt);
 }
 
 int dgc_rk4_profile_stages(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,

@@ -82,6 +82,9 @@ def _declare(L):
                        C.POINTER(C.c_float)])
     d("dgc_ipdg_pcg_mixed", [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, _ip, _ip, _dp,
                              C.POINTER(C.c_float)])
+    d("dgc_op_set_waves_per_tile", [_vp, C.c_int])
+    d("dgc_op_waves_per_tile", [_vp])
+    d("dgc_op_autotune", [_vp, C.POINTER(_vp), C.c_size_t, C.POINTER(_vp), C.c_int, C.POINTER(C.c_float)])
     for name, args in _EXTRA_DECLS:
         d(name, args)
 
@@ -385,6 +388,21 @@ class _Operator:
         check(lib().dgc_rk4_steps_timed(self._h, self._fp(state), self.disc.size, t0, dt, nsteps,
                                         _ptrs([w.ptr for w in work]), C.byref(ms)))
         return ms.value
+
+    @property
+    def waves_per_tile(self) -> int:
+        return lib().dgc_op_waves_per_tile(self._h)
+
+    @waves_per_tile.setter
+    def waves_per_tile(self, wt: int):
+        check(lib().dgc_op_set_waves_per_tile(self._h, wt))
+
+    def autotune(self, state, work, reps=20) -> dict:
+        """Time every built tiling on these buffers and keep the fastest."""
+        ms = (C.c_float * 3)()
+        check(lib().dgc_op_autotune(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
+                                    reps, ms))
+        return dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4), ms) if m >= 0})
 
     def profile_stages(self, state, t0, dt, nsteps, work):
         """Mean duration (ms) of each of the 4 stage kernels over nsteps steps."""

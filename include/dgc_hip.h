@@ -117,6 +117,17 @@ int dgc_advection_create(dgc_disc_t disc, const double* Dr, const double* Ds, co
 int dgc_op_destroy(dgc_op_t op);
 int dgc_op_num_fields(dgc_op_t op);
 
+/* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4; the
+ * reference's launch-shape arguments, main.cpp argv[2], play this role).
+ * A tiling whose LDS would not fit falls back to 4. dgc_op_autotune times
+ * every built tiling on the caller's buffers (work vectors only are written)
+ * and keeps the fastest; ms_out[3] receives the mean stage-2 kernel time of
+ * wt = 1, 2, 4 (-1 where not built). No reference counterpart. */
+int dgc_op_set_waves_per_tile(dgc_op_t op, int wt);
+int dgc_op_waves_per_tile(dgc_op_t op);
+int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
+                    float* ms_out);
+
 /* out = L_h(in, t): one evaluation of the semi-discrete operator.
  * Replaces device_SemiDiscOp_wrapper::operator() (src/device_SemiDiscOp_wrapper.cuh:45-58).
  * in/out: arrays of F device pointers (F = 3 for Maxwell: Hx, Hy, Ez), each K*Np. */

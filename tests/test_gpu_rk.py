@@ -199,3 +199,37 @@ def test_bad_arguments_are_errors(ctx, c1):
                                        _ptrs([x.ptr for x in a])))
     with pytest.raises(dgc_amd.DGCError):
         op.rk4_run(a, 0.0, 1.0, -1.0, op.new_work())
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5])
+def test_every_tiling_gives_identical_results(ctx, N):
+    """The stage kernels' tilings (waves per tile) agree to rounding (the
+    compiler may contract multiply-adds differently per instantiation)."""
+    mesh = dgc_amd.Mesh.structured(20, 13, 0, 1, 0, 1, jitter=0.05, seed=4, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    results = []
+    for phys in ("adv", "mx"):
+        for wt in (1, 2, 4):
+            op = (dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO) if phys == "adv"
+                  else dgc_amd.MaxwellTM(disc))
+            op.waves_per_tile = wt
+            F = op.F
+            u0 = np.random.default_rng(N).standard_normal(F * disc.size)
+            st = [dgc_amd.DeviceArray.from_host(ctx, f) for f in np.split(u0, F)]
+            op.rk4_steps(st, 0.0, 1e-4, 3, op.new_work())
+            results.append((phys, op.waves_per_tile, np.concatenate([a.download() for a in st])))
+    for phys in ("adv", "mx"):
+        rs = [r for p, _, r in results if p == phys]
+        for r in rs[1:]:
+            assert rel(r, rs[0]) < 1e-14
+
+
+def test_autotune_leaves_state_untouched(ctx):
+    mesh = dgc_amd.Mesh.structured(32, 32, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, 2)
+    op = dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO)
+    u0 = np.random.default_rng(0).standard_normal(disc.size)
+    st = [dgc_amd.DeviceArray.from_host(ctx, u0)]
+    res = op.autotune(st, op.new_work(), reps=3)
+    assert res["choice"] in (1, 2, 4) and res["ms"]
+    np.testing.assert_array_equal(st[0].download(), u0)
