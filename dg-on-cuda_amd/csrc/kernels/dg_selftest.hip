@@ -1,0 +1,60 @@
+// Device self-test of the MFMA operand layouts documented in dg_mfma.hpp.
+// One wave multiplies caller-given matrices with the builtin and scatters
+// the result with the documented layout; the caller compares with A*B.
+#include <string>
+
+#include "../../../include/dgc_hip.h"
+#include "dg_internal.hpp"
+#include "dg_mfma.hpp"
+
+namespace {
+
+__global__ void mfma_bf16_probe(const float* A, const float* B, float* C) {
+  // A: 32x8 row-major, B: 8x32 row-major, C: 32x32 row-major
+  const int l = threadIdx.x;
+  dgc::bf16x4 a, b;
+  for (int v = 0; v < 4; ++v) {
+    a[v] = short(dgc::f2bf(A[(l % 32) * 8 + 4 * (l / 32) + v]));
+    b[v] = short(dgc::f2bf(B[(4 * (l / 32) + v) * 32 + l % 32]));
+  }
+  dgc::floatx16 d = {};
+  d = dgc::mfma32x32x8bf16(a, b, d);
+  for (int r = 0; r < 16; ++r) C[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = d[r];
+}
+
+__global__ void mfma_f32_probe(const float* A, const float* B, float* C) {
+  // A: 16x4, B: 4x16, C: 16x16, row-major
+  const int l = threadIdx.x;
+  dgc::floatx4 d = {};
+  d = dgc::mfma16x16x4f32(A[(l % 16) * 4 + l / 16], B[(l / 16) * 16 + l % 16], d);
+  for (int r = 0; r < 4; ++r) C[(4 * (l / 16) + r) * 16 + l % 16] = d[r];
+}
+
+}  // namespace
+
+extern "C" int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C) {
+  if (!ctx || !A || !B || !C || (kind != 0 && kind != 1))
+    return dgc_set_error(DGC_ERR_INVALID, "dgc_selftest_mfma: bad argument");
+  const int na = kind == 0 ? 32 * 8 : 16 * 4, nb = kind == 0 ? 8 * 32 : 4 * 16, nc = kind == 0 ? 32 * 32 : 16 * 16;
+  float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  hipError_t e = hipMalloc(&dA, na * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dB, nb * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dC, nc * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dA, A, na * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dB, B, nb * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(dC, 0, nc * sizeof(float));
+  if (e == hipSuccess) {
+    if (kind == 0)
+      hipLaunchKernelGGL(mfma_bf16_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    else
+      hipLaunchKernelGGL(mfma_f32_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(C, dC, nc * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("dgc_selftest_mfma: ") + hipGetErrorString(e));
+  return DGC_OK;
+}

@@ -198,6 +198,12 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
                        int max_outer, int max_inner, int* outer_iters, int* inner_iters,
                        double* relres, float* ms);
 
+/* Device self-test of the matrix-core operand layouts the kernels rely on:
+ * kind 0: C[32x32] = A[32x8] B[8x32] with bf16 inputs (rounded on the device),
+ * kind 1: C[16x16] = A[16x4] B[4x16] in fp32. Row-major host arrays; the
+ * result is scattered with the documented lane layout. No reference counterpart. */
+int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C);
+
 /* Same as dgc_rk4_steps, bracketed by HIP events on the context stream;
  * blocks until done and returns the elapsed milliseconds in *ms
  * (measurement entry point; no reference counterpart). */
