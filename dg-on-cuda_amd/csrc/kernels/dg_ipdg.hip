@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
       for (int k = 0; k < Np; ++k) z = fma(p.P[((size_t)j * Np + k) * K + c], rs[k][lane], z);
     }
     if (valid) p.z[(size_t)j * K + c] = z;
-    rz += rs[j][lane] * z;
+    rz += (p.zz ? z : rs[j][lane]) * z;
   }
   T s1 = block_sum(rz, red);
   __syncthreads();
@@ -401,6 +401,7 @@ __global__ void cg_scalars(const T* __restrict__ partial, int nparts, int which,
   }
   if (scal[kDone] != T(0)) return;
   if (which == kAfterApply) {
+    scal[kPq] = T(sh[0][0]);
     scal[kAlpha] = T(double(scal[kRz]) / sh[0][0]);
   } else if (which == kAfterUpdate) {
     const double rz = sh[0][0], rr = sh[1][0];

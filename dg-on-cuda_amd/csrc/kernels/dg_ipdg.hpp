@@ -8,7 +8,11 @@ namespace dgc {
 // kPhaseOne / kPhaseTwo: one pass only (the reference's two-phase launch).
 enum IpdgMode { kLaplace = 0, kSpd = 1, kPhaseOne = 2, kPhaseTwo = 3 };
 // CG scalar slots (device array of 8 values of the solver's type)
-enum CgScalar { kAlpha = 0, kBeta = 1, kRz = 2, kRr = 3, kIter = 4, kDone = 5, kTol2 = 6, kMaxit = 7, kCgScalars = 8 };
+enum CgScalar {
+  kAlpha = 0, kBeta = 1, kRz = 2, kRr = 3, kIter = 4, kDone = 5, kTol2 = 6, kMaxit = 7,
+  kPq = 8,  // p.q of the current iteration (flexible CG's beta)
+  kCgScalars = 10
+};
 enum CgStep { kInit = 0, kAfterApply = 1, kAfterUpdate = 2 };
 
 template <typename T>
@@ -42,7 +46,8 @@ struct CgParams {
   T* z;
   const T* P;     // block-Jacobi inverse blocks [Np*Np][K] or null
   const T* scal;  // CG scalars
-  T* partial;     // [2 * grid]
+  T* partial;     // [2 * grid]: (r.z, r.r), or (z.z, r.r) when zz != 0
+  int zz = 0;
 };
 
 template <typename T>

@@ -1,0 +1,539 @@
+// Learned preconditioner kernels (network defined in dg_mpnn.hpp).
+//
+// Forward layers run on the matrix cores: a wave owns 32 cells and computes
+// their 64 hidden units as two 32x32 tiles of v_mfma_f32_32x32x8bf16_1k over
+// the concatenated [own | neighbour-sum] inputs; the layer's weights are
+// pre-packed in the B-operand order and staged in LDS once per workgroup.
+// Backward and weight-gradient kernels read the same bf16 data and
+// accumulate in fp32; weight gradients are reduced per workgroup into a
+// partial array and summed in a fixed order (deterministic training).
+#include <algorithm>
+#include <cmath>
+
+#include "dg_ipdg.hpp"
+#include "dg_mfma.hpp"
+#include "dg_mpnn.hpp"
+
+namespace dgc {
+
+int mpnn_grid(int K) { return (K + kMpnnCells - 1) / kMpnnCells; }
+
+namespace {
+
+__device__ __forceinline__ bf16x4 as_bf16x4(uint64_t u) {
+  bf16x4 r;
+  __builtin_memcpy(&r, &u, 8);
+  return r;
+}
+
+__device__ __forceinline__ void load_bf16x4(const uint16_t* p, float out[4]) {
+  uint64_t u = *reinterpret_cast<const uint64_t*>(p);
+  for (int v = 0; v < 4; ++v) out[v] = bf2f(uint16_t(u >> (16 * v)));
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum256(T v, T* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < 4; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// ---- features ---------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void features_kernel(MpnnNet net, const double* __restrict__ r,
+                                                       const double* __restrict__ zb, const double* __restrict__ geo,
+                                                       const float* __restrict__ gnorm, const double* dscal,
+                                                       const double* cgscal, uint16_t* __restrict__ X) {
+  constexpr int Np = Tri<N>::Np;
+  if (cgscal && cgscal[kDone] != 0.0) return;
+  const int K = net.K, D0 = net.L.D0;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= K) return;
+  const double isr = dscal[kSr] > 0 ? 1.0 / dscal[kSr] : 0.0;
+  const double isz = dscal[kSz] > 0 ? 1.0 / dscal[kSz] : 0.0;
+  uint16_t* x = X + (size_t)c * D0;
+#pragma unroll
+  for (int j = 0; j < Np; ++j) {
+    x[j] = f2bf(float(r[(size_t)j * K + c] * isr));
+    x[Np + j] = f2bf(float(zb[(size_t)j * K + c] * isz));
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) x[2 * Np + k] = f2bf(float(geo[(size_t)k * K + c]) / gnorm[k]);
+  for (int k = 2 * Np + 4; k < D0; ++k) x[k] = 0;
+}
+
+// ---- dense layer forward (MFMA) ---------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, const uint16_t* __restrict__ hin,
+                                                        uint16_t* __restrict__ hout, const double* cgscal) {
+  constexpr int KS = D / 4;  // k-steps of 8 over [own (D) | neighbour sum (D)]
+  __shared__ uint64_t wl[KS * 2 * 64];
+  if (cgscal && cgscal[kDone] != 0.0) return;
+  const uint64_t* wp = net.wpk[layer];
+  for (int i = threadIdx.x; i < KS * 2 * 64; i += 256) wl[i] = wp[i];
+  __syncthreads();
+  const int K = net.K;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 31, kq = lane >> 5;
+  const int cbase = blockIdx.x * kMpnnCells + wave * 32;
+  const int c = cbase + row;
+  const bool valid = c < K;
+  int nb[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int code = valid ? net.nbr[(size_t)e * K + c] : kBoundaryCode;
+    nb[e] = (code & 3) != kBoundaryCode ? (code >> 2) : -1;
+  }
+  floatx16 acc0 = {}, acc1 = {};
+#pragma unroll
+  for (int ks = 0; ks < D / 8; ++ks) {
+    uint64_t u = 0;
+    if (valid) u = *reinterpret_cast<const uint64_t*>(hin + (size_t)c * D + ks * 8 + 4 * kq);
+    const bf16x4 a = as_bf16x4(u);
+    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 0) * 64 + lane]), acc0);
+    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 1) * 64 + lane]), acc1);
+  }
+#pragma unroll
+  for (int ks = 0; ks < D / 8; ++ks) {
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (nb[e] >= 0) {
+        float t[4];
+        load_bf16x4(hin + (size_t)nb[e] * D + ks * 8 + 4 * kq, t);
+        for (int v = 0; v < 4; ++v) s[v] += t[v];
+      }
+    }
+    uint64_t u = 0;
+    for (int v = 0; v < 4; ++v) u |= uint64_t(f2bf(s[v])) << (16 * v);
+    const bf16x4 a = as_bf16x4(u);
+    const int kk = D / 8 + ks;
+    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 0) * 64 + lane]), acc0);
+    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 1) * 64 + lane]), acc1);
+  }
+  const float* bias = net.wf + net.L.ob[layer];
+  const int j = lane & 31;
+  const float b0 = bias[j], b1 = bias[32 + j];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int cc = cbase + 8 * (r / 4) + 4 * kq + (r % 4);
+    if (cc < K) {
+      hout[(size_t)cc * kHid + j] = f2bf(fmaxf(acc0[r] + b0, 0.f));
+      hout[(size_t)cc * kHid + 32 + j] = f2bf(fmaxf(acc1[r] + b1, 0.f));
+    }
+  }
+}
+
+// ---- output layer -------------------------------------------------------------
+template <int N, int MODE>
+__global__ __launch_bounds__(256) void out_kernel(MpnnNet net, MpnnOutArgs a, const double* cgscal) {
+  constexpr int Np = Tri<N>::Np;
+  __shared__ double red[4];
+  if (cgscal && cgscal[kDone] != 0.0) return;
+  const int K = net.K;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = c < K;
+  float h[kHid];
+#pragma unroll
+  for (int q4 = 0; q4 < kHid / 4; ++q4) {
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) load_bf16x4(a.h3 + (size_t)c * kHid + 4 * q4, t);
+    for (int v = 0; v < 4; ++v) h[4 * q4 + v] = t[v];
+  }
+  const uint16_t* Wo = net.wbf + net.L.oWo;
+  const float* bo = net.wf + net.L.obo;
+  float y[Np];
+#pragma unroll
+  for (int j = 0; j < Np; ++j) {
+    float acc = bo[j];
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) acc = fmaf(bf2f(Wo[j * kHid + o]), h[o], acc);
+    y[j] = acc;
+  }
+  if (MODE == kOutY) {
+    if (valid)
+      for (int j = 0; j < Np; ++j) a.y[(size_t)c * Np + j] = y[j];
+    return;
+  }
+  if (MODE == kOutPrecond) {
+    const double scale = a.dscal[kSz] * a.dscal[kCt];
+    double rz = 0.0, zq = 0.0;
+    if (valid) {
+#pragma unroll
+      for (int j = 0; j < Np; ++j) {
+        const size_t idx = (size_t)j * K + c;
+        const double z = a.zb[idx] + scale * double(y[j]);
+        a.z[idx] = z;
+        if (a.r) rz += a.r[idx] * z;
+        if (a.q) zq += z * a.q[idx];
+      }
+    }
+    if (a.partial) {
+      const double s1 = block_sum256(rz, red);
+      const double s2 = block_sum256(zq, red);
+      if (threadIdx.x == 0) {
+        a.partial[2 * blockIdx.x] = s1;
+        a.partial[2 * blockIdx.x + 1] = s2;
+      }
+    }
+    return;
+  }
+  // training: mean-square loss, dL/dy and dL/dh3
+  double loss = 0.0;
+  float dy[Np];
+#pragma unroll
+  for (int j = 0; j < Np; ++j) {
+    const float d = valid ? y[j] - a.t[(size_t)c * Np + j] : 0.f;
+    loss += double(d) * double(d);
+    dy[j] = float(2.0 * double(d) * a.inv_n);
+    if (valid) a.dy[(size_t)c * Np + j] = dy[j];
+  }
+  if (valid) {
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) {
+      float acc = 0.f;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) acc = fmaf(bf2f(Wo[j * kHid + o]), dy[j], acc);
+      a.dh3[(size_t)c * kHid + o] = acc;
+    }
+  }
+  const double s = block_sum256(loss, red);
+  if (threadIdx.x == 0) a.loss_partial[blockIdx.x] = s;
+}
+
+// ---- energy-norm loss gradient ----------------------------------------------
+// z = zb + s y, w = S z: dL/dy = s * 2 (w - r) / (e.r); also dL/dh3 and the
+// loss terms z.w and z.r per workgroup.
+template <int N>
+__global__ __launch_bounds__(256) void energy_grad_kernel(MpnnNet net, const double* __restrict__ z,
+                                                          const double* __restrict__ w,
+                                                          const double* __restrict__ r,
+                                                          const double* __restrict__ mscal, double er,
+                                                          float* __restrict__ dy, float* __restrict__ dh3,
+                                                          double* __restrict__ partial) {
+  constexpr int Np = Tri<N>::Np;
+  __shared__ double red[4];
+  const int K = net.K;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = c < K;
+  const double g = 2.0 * mscal[kSz] * mscal[kCt] / er;
+  const uint16_t* Wo = net.wbf + net.L.oWo;
+  double zw = 0.0, zr = 0.0;
+  float d[Np];
+#pragma unroll
+  for (int j = 0; j < Np; ++j) {
+    d[j] = 0.f;
+    if (valid) {
+      const size_t idx = (size_t)j * K + c;
+      const double zi = z[idx], wi = w[idx], ri = r[idx];
+      zw += zi * wi;
+      zr += zi * ri;
+      d[j] = float(g * (wi - ri));
+      dy[(size_t)c * Np + j] = d[j];
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) {
+      float acc = 0.f;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) acc = fmaf(bf2f(Wo[j * kHid + o]), d[j], acc);
+      dh3[(size_t)c * kHid + o] = acc;
+    }
+  }
+  const double s1 = block_sum256(zw, red);
+  const double s2 = block_sum256(zr, red);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s1;
+    partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// ---- dense layer backward ---------------------------------------------------
+// dz = dh * (h > 0) (in place in dh); u = A^T dz, v = B^T dz for the layer below.
+template <int D>
+__global__ __launch_bounds__(256) void dense_bwd_kernel(MpnnNet net, int layer, const uint16_t* __restrict__ hl,
+                                                        float* __restrict__ dh, float* __restrict__ u,
+                                                        float* __restrict__ v) {
+  const int K = net.K;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= K) return;
+  float dz[kHid];
+#pragma unroll
+  for (int q4 = 0; q4 < kHid / 4; ++q4) {
+    float t[4];
+    load_bf16x4(hl + (size_t)c * kHid + 4 * q4, t);
+    for (int w = 0; w < 4; ++w) {
+      const int o = 4 * q4 + w;
+      const float g = dh[(size_t)c * kHid + o];
+      dz[o] = t[w] > 0.f ? g : 0.f;
+      dh[(size_t)c * kHid + o] = dz[o];
+    }
+  }
+  if (!u) return;
+  const uint16_t* A = net.wbf + net.L.oA[layer];
+  const uint16_t* B = net.wbf + net.L.oB[layer];
+  for (int i = 0; i < D; ++i) {
+    float su = 0.f, sv = 0.f;
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) {
+      su = fmaf(bf2f(A[o * D + i]), dz[o], su);
+      sv = fmaf(bf2f(B[o * D + i]), dz[o], sv);
+    }
+    u[(size_t)c * D + i] = su;
+    v[(size_t)c * D + i] = sv;
+  }
+}
+
+// dhprev_c = u_c + sum_{n in N(c)} v_n (adjoint of the neighbour sum)
+__global__ __launch_bounds__(256) void nbr_accum_kernel(MpnnNet net, int D, const float* __restrict__ u,
+                                                        const float* __restrict__ v, float* __restrict__ dhp) {
+  const int K = net.K;
+  const size_t n = size_t(K) * D;
+  const size_t stride = size_t(blockDim.x) * gridDim.x;
+  for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n; idx += stride) {
+    const int c = int(idx / D), i = int(idx % D);
+    float s = u[idx];
+    for (int e = 0; e < 3; ++e) {
+      const int code = net.nbr[(size_t)e * K + c];
+      if ((code & 3) != kBoundaryCode) s += v[(size_t)(code >> 2) * D + i];
+    }
+    dhp[idx] = s;
+  }
+}
+
+// ---- weight gradients ---------------------------------------------------------
+template <int DZ, int DX>
+__global__ __launch_bounds__(256) void wgrad_kernel(MpnnNet net, const float* __restrict__ dz,
+                                                    const uint16_t* __restrict__ hin, int gather,
+                                                    float* __restrict__ partial, int pstride, int off, int boff) {
+  constexpr int CH = 32;
+  constexpr int NOUT = DZ * DX;
+  constexpr int PER = (NOUT + 255) / 256;
+  __shared__ float zs[CH][DZ];
+  __shared__ float xs[CH][DX];
+  const int K = net.K;
+  const int c0 = blockIdx.x * kMpnnCells;
+  float acc[PER];
+  for (int p = 0; p < PER; ++p) acc[p] = 0.f;
+  float bacc = 0.f;
+  for (int ch = 0; ch < kMpnnCells; ch += CH) {
+    for (int idx = threadIdx.x; idx < CH * DZ; idx += 256) {
+      const int cc = idx / DZ, o = idx % DZ, c = c0 + ch + cc;
+      zs[cc][o] = c < K ? dz[(size_t)c * DZ + o] : 0.f;
+    }
+    for (int idx = threadIdx.x; idx < CH * DX; idx += 256) {
+      const int cc = idx / DX, i = idx % DX, c = c0 + ch + cc;
+      float val = 0.f;
+      if (c < K) {
+        if (!gather) {
+          val = bf2f(hin[(size_t)c * DX + i]);
+        } else {
+          float s = 0.f;
+          for (int e = 0; e < 3; ++e) {
+            const int code = net.nbr[(size_t)e * K + c];
+            if ((code & 3) != kBoundaryCode) s += bf2f(hin[(size_t)(code >> 2) * DX + i]);
+          }
+          val = bf2f(f2bf(s));  // the forward pass rounds the neighbour sum to bf16
+        }
+      }
+      xs[cc][i] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = threadIdx.x + 256 * p;
+      if (idx < NOUT) {
+        const int o = idx / DX, i = idx % DX;
+        float s = acc[p];
+        for (int cc = 0; cc < CH; ++cc) s = fmaf(zs[cc][o], xs[cc][i], s);
+        acc[p] = s;
+      }
+    }
+    if (boff >= 0 && threadIdx.x < DZ)
+      for (int cc = 0; cc < CH; ++cc) bacc += zs[cc][threadIdx.x];
+    __syncthreads();
+  }
+  float* out = partial + (size_t)blockIdx.x * pstride;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int idx = threadIdx.x + 256 * p;
+    if (idx < NOUT) out[off + idx] = acc[p];
+  }
+  if (boff >= 0 && threadIdx.x < DZ) out[boff + threadIdx.x] = bacc;
+}
+
+__global__ void reduce_kernel(const float* __restrict__ partial, int nblocks, int pstride, int P,
+                              float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  float s = 0.f;
+  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * pstride + p];
+  grad[p] = s;
+}
+
+__global__ void adam_kernel(float* w, float* m, float* v, const float* g, int P, float lr, float b1, float b2,
+                            float eps, float c1, float c2, uint16_t* wbf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const float gi = g[p];
+  const float mi = b1 * m[p] + (1.f - b1) * gi;
+  const float vi = b2 * v[p] + (1.f - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  const float wi = w[p] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[p] = wi;
+  wbf[p] = f2bf(wi);
+}
+
+// packed[(ks*2 + tile)*64 + lane] = 4 bf16 of Wcat^T[ks*8 + 4*(lane/32) + v][tile*32 + lane%32]
+__global__ void pack_kernel(MpnnLayout L, int layer, const uint16_t* __restrict__ wbf, uint64_t* __restrict__ out) {
+  const int D = L.din(layer);
+  const int KS = D / 4;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= KS * 2 * 64) return;
+  const int lane = idx % 64, tile = (idx / 64) % 2, ks = idx / 128;
+  const int o = tile * 32 + lane % 32;
+  uint64_t u = 0;
+  for (int v = 0; v < 4; ++v) {
+    const int k = ks * 8 + 4 * (lane / 32) + v;
+    const uint16_t w = k < D ? wbf[L.oA[layer] + o * D + k] : wbf[L.oB[layer] + o * D + (k - D)];
+    u |= uint64_t(w) << (16 * v);
+  }
+  out[idx] = u;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+#define DGC_N_SWITCH(order, CALL) \
+  switch (order) {                \
+    case 1: CALL(1); break;       \
+    case 2: CALL(2); break;       \
+    case 3: CALL(3); break;       \
+    case 4: CALL(4); break;       \
+    case 5: CALL(5); break;       \
+    case 6: CALL(6); break;       \
+    case 7: CALL(7); break;       \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t mpnn_features(int order, const MpnnNet& net, const double* r, const double* zb, const double* geo,
+                         const float* gnorm, const double* dscal, uint16_t* X, hipStream_t s,
+                         const double* cgscal) {
+  const int grid = (net.K + 255) / 256;
+#define CALL(n) hipLaunchKernelGGL(features_kernel<n>, dim3(grid), dim3(256), 0, s, net, r, zb, geo, gnorm, dscal, cgscal, X)
+  DGC_N_SWITCH(order, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t mpnn_dense_fwd(const MpnnNet& net, int layer, const uint16_t* hin, uint16_t* hout, hipStream_t s,
+                          const double* cgscal) {
+  const int D = net.L.din(layer);
+  const dim3 grid(mpnn_grid(net.K));
+  switch (D) {
+    case 16: hipLaunchKernelGGL(dense_fwd_kernel<16>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    case 24: hipLaunchKernelGGL(dense_fwd_kernel<24>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    case 40: hipLaunchKernelGGL(dense_fwd_kernel<40>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    case 48: hipLaunchKernelGGL(dense_fwd_kernel<48>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    case 64: hipLaunchKernelGGL(dense_fwd_kernel<64>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    case 80: hipLaunchKernelGGL(dense_fwd_kernel<80>, grid, dim3(256), 0, s, net, layer, hin, hout, cgscal); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t mpnn_out(int order, const MpnnNet& net, int mode, const MpnnOutArgs& a, hipStream_t s,
+                    const double* cgscal) {
+  const int grid = (net.K + 255) / 256;
+#define CALL(n)                                                                                    \
+  if (mode == kOutY)                                                                               \
+    hipLaunchKernelGGL((out_kernel<n, kOutY>), dim3(grid), dim3(256), 0, s, net, a, cgscal);       \
+  else if (mode == kOutPrecond)                                                                    \
+    hipLaunchKernelGGL((out_kernel<n, kOutPrecond>), dim3(grid), dim3(256), 0, s, net, a, cgscal); \
+  else                                                                                             \
+    hipLaunchKernelGGL((out_kernel<n, kOutTrain>), dim3(grid), dim3(256), 0, s, net, a, cgscal)
+  DGC_N_SWITCH(order, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t mpnn_energy_grad(int order, const MpnnNet& net, const double* z, const double* w, const double* r,
+                            const double* mscal, double er, float* dy, float* dh3, double* partial, hipStream_t s) {
+  const int grid = (net.K + 255) / 256;
+#define CALL(n) \
+  hipLaunchKernelGGL(energy_grad_kernel<n>, dim3(grid), dim3(256), 0, s, net, z, w, r, mscal, er, dy, dh3, partial)
+  DGC_N_SWITCH(order, CALL)
+#undef CALL
+  return hipGetLastError();
+}
+
+hipError_t mpnn_dense_bwd(const MpnnNet& net, int layer, const uint16_t* hl, float* dh, float* u, float* v,
+                          hipStream_t s) {
+  // layer is the index of h^{layer+1}'s weights; u, v have the width of its input
+  const int D = net.L.din(layer);
+  const dim3 grid((net.K + 255) / 256);
+  switch (D) {
+    case 16: hipLaunchKernelGGL(dense_bwd_kernel<16>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    case 24: hipLaunchKernelGGL(dense_bwd_kernel<24>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    case 40: hipLaunchKernelGGL(dense_bwd_kernel<40>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    case 48: hipLaunchKernelGGL(dense_bwd_kernel<48>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    case 64: hipLaunchKernelGGL(dense_bwd_kernel<64>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    case 80: hipLaunchKernelGGL(dense_bwd_kernel<80>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t mpnn_nbr_accum(const MpnnNet& net, int din, const float* u, const float* v, float* dhprev, hipStream_t s) {
+  const size_t n = size_t(net.K) * din;
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 16384));
+  hipLaunchKernelGGL(nbr_accum_kernel, dim3(grid), dim3(256), 0, s, net, din, u, v, dhprev);
+  return hipGetLastError();
+}
+
+hipError_t mpnn_wgrad(const MpnnNet& net, int dz_width, const float* dz, int dx_width, const uint16_t* hin,
+                      bool gather, float* partial, int pstride, int off, int boff, hipStream_t s) {
+  const dim3 grid(mpnn_grid(net.K));
+  const int g = gather ? 1 : 0;
+#define WG(DZ, DX)                                                                                            \
+  if (dz_width == DZ && dx_width == DX) {                                                                     \
+    hipLaunchKernelGGL((wgrad_kernel<DZ, DX>), grid, dim3(256), 0, s, net, dz, hin, g, partial, pstride, off, \
+                       boff);                                                                                 \
+    return hipGetLastError();                                                                                 \
+  }
+  WG(64, 16) WG(64, 24) WG(64, 40) WG(64, 48) WG(64, 64) WG(64, 80)
+  WG(3, 64) WG(6, 64) WG(10, 64) WG(15, 64) WG(21, 64) WG(28, 64) WG(36, 64)
+#undef WG
+  return hipErrorInvalidValue;
+}
+
+hipError_t mpnn_reduce(const float* partial, int nblocks, int pstride, int P, float* grad, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_kernel, dim3((P + 255) / 256), dim3(256), 0, s, partial, nblocks, pstride, P, grad);
+  return hipGetLastError();
+}
+
+hipError_t mpnn_adam(float* w, float* m, float* v, const float* g, int P, float lr, float b1, float b2, float eps,
+                     int step, uint16_t* wbf, hipStream_t s) {
+  const float c1 = 1.f - std::pow(b1, float(step));
+  const float c2 = 1.f - std::pow(b2, float(step));
+  hipLaunchKernelGGL(adam_kernel, dim3((P + 255) / 256), dim3(256), 0, s, w, m, v, g, P, lr, b1, b2, eps, c1, c2,
+                     wbf);
+  return hipGetLastError();
+}
+
+hipError_t mpnn_pack(const MpnnLayout& L, const uint16_t* wbf, uint64_t* const* wpk, hipStream_t s) {
+  for (int l = 0; l < 3; ++l) {
+    const int n = L.din(l) / 4 * 2 * 64;
+    hipLaunchKernelGGL(pack_kernel, dim3((n + 255) / 256), dim3(256), 0, s, L, l, wbf, wpk[l]);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dgc

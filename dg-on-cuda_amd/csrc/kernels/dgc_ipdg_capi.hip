@@ -287,6 +287,15 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
   return DGC_OK;
 }
 
+// FP64 PCG restart from the current iterate (used by the flexible-CG fallback).
+int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, int maxit, double* history) {
+  return cg_init<double>(s, s->f64, b, tol2_times_bb, maxit, false, history);
+}
+
+int cg_run_fp64(dgc_ipdg_s* s, double* history, int maxit, int* iters) {
+  return cg_run<double>(s, s->f64, history, maxit, iters);
+}
+
 static void face_penalty(const dgc_disc_s* d, int tau_mode, double tau_scale, std::vector<double>& tau) {
   const int K = d->K, N = d->order;
   tau.assign(3 * size_t(K), tau_scale);

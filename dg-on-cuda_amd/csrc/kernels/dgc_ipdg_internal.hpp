@@ -50,3 +50,7 @@ struct dgc_ipdg_s {
   IpdgLevel<double> f64;
   IpdgLevel<float> f32;
 };
+
+// FP64 block-Jacobi PCG continuing from the current iterate f64.x.
+int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, int maxit, double* history);
+int cg_run_fp64(dgc_ipdg_s* s, double* history, int maxit, int* iters);

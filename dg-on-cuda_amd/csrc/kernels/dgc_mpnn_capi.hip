@@ -1,0 +1,703 @@
+// C ABI of the learned preconditioner (include/dgc_hip.h): network state,
+// training data from the IPDG operator itself, GPU training with the
+// project's forward/backward kernels, and flexible CG with the network and
+// a block-Jacobi fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../../include/dgc_hip.h"
+#include "../host/mesh.hpp"
+#include "dg_internal.hpp"
+#include "dg_ipdg.hpp"
+#include "dg_mfma.hpp"
+#include "dg_mpnn.hpp"
+#include "dgc_ipdg_internal.hpp"
+
+#define DGC_HIP_TRY(call)                                                                     \
+  do {                                                                                        \
+    hipError_t err_ = (call);                                                                 \
+    if (err_ != hipSuccess)                                                                   \
+      return dgc_set_error(DGC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+#define DGC_REQUIRE(cond, msg) \
+  do {                         \
+    if (!(cond)) return dgc_set_error(DGC_ERR_INVALID, msg); \
+  } while (0)
+#define DGC_LAUNCH(call)                                                                                   \
+  do {                                                                                                     \
+    hipError_t err_ = (call);                                                                              \
+    if (err_ != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+struct dgc_mpnn_s {
+  dgc_ipdg_s* s = nullptr;
+  dgc::MpnnLayout L;
+  int K = 0, N = 0, Np = 0;
+  float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  uint16_t* wbf = nullptr;
+  uint64_t* wpk[3] = {nullptr, nullptr, nullptr};
+  float* partial = nullptr;
+  int pstride = 0, nblk = 0, out_grid = 0, cg_grid = 0;
+  float* gnorm = nullptr;
+  double* mscal = nullptr;
+  double* skip_scal = nullptr;  // CG scalars with the done flag set (z = P r only)
+  double* tmp_scal = nullptr;   // scratch scalars for reductions outside a solve
+  uint16_t* X = nullptr;
+  uint16_t* H[3] = {nullptr, nullptr, nullptr};
+  float *dh = nullptr, *dh2 = nullptr, *u = nullptr, *vv = nullptr, *dy = nullptr, *y = nullptr;
+  float* tgt = nullptr;
+  double* loss_partial = nullptr;
+  double* partial_out = nullptr;
+  double* zfull = nullptr;
+  double* dof_x = nullptr;
+  double* dof_y = nullptr;
+  double* e = nullptr;
+  double* amp = nullptr;
+  int nsamples = 0;
+  double* ds_r = nullptr;         // training residuals r = S e, [nsamples][Np][K]
+  std::vector<double> ds_er;      // e . r per sample
+  double* wbuf = nullptr;         // S z
+  double* dotpart = nullptr;      // [256] + 1 result slot
+  int adam_step = 0;
+  double ct = 1.0;
+  hipGraphExec_t graph = nullptr;
+  double* graph_hist = nullptr;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(dgc_mpnn_s* m, T** p, size_t n) {
+  DGC_HIP_TRY(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(T)));
+  m->owned.push_back(*p);
+  return DGC_OK;
+}
+
+dgc::MpnnNet net_of(const dgc_mpnn_s* m) {
+  dgc::MpnnNet n;
+  n.L = m->L;
+  n.K = m->K;
+  n.nbr = m->s->disc->nbr;
+  n.wbf = m->wbf;
+  n.wf = m->w;
+  for (int l = 0; l < 3; ++l) n.wpk[l] = m->wpk[l];
+  return n;
+}
+
+hipStream_t stream_of(const dgc_mpnn_s* m) { return m->s->disc->ctx->stream; }
+
+// Normal deviates from splitmix64 (Box-Muller), deterministic per seed.
+struct Gauss {
+  dgc_host::SplitMix64 rng;
+  explicit Gauss(uint64_t seed) : rng(seed) {}
+  double operator()() {
+    double u1 = rng.uniform(), u2 = rng.uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+int upload_weights(dgc_mpnn_s* m, const std::vector<float>& w) {
+  std::vector<uint16_t> wb(w.size());
+  for (size_t i = 0; i < w.size(); ++i) wb[i] = dgc::f2bf(w[i]);
+  hipStream_t st = stream_of(m);
+  DGC_HIP_TRY(hipMemcpyAsync(m->w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipMemcpyAsync(m->wbf, wb.data(), wb.size() * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipMemsetAsync(m->m, 0, w.size() * sizeof(float), st));
+  DGC_HIP_TRY(hipMemsetAsync(m->v, 0, w.size() * sizeof(float), st));
+  DGC_LAUNCH(dgc::mpnn_pack(m->L, m->wbf, m->wpk, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  m->adam_step = 0;
+  return DGC_OK;
+}
+
+int forward(dgc_mpnn_s* m, const uint16_t* X, const double* cgscal = nullptr) {
+  const dgc::MpnnNet net = net_of(m);
+  hipStream_t st = stream_of(m);
+  DGC_LAUNCH(dgc::mpnn_dense_fwd(net, 0, X, m->H[0], st, cgscal));
+  DGC_LAUNCH(dgc::mpnn_dense_fwd(net, 1, m->H[0], m->H[1], st, cgscal));
+  DGC_LAUNCH(dgc::mpnn_dense_fwd(net, 2, m->H[1], m->H[2], st, cgscal));
+  return DGC_OK;
+}
+
+int backward_from(dgc_mpnn_s* m, const uint16_t* X);
+
+// forward + mean-square loss + backward for one sample; gradient in m->grad,
+// loss partials in m->loss_partial
+int loss_grad(dgc_mpnn_s* m, const uint16_t* X, const float* T) {
+  int rc = forward(m, X);
+  if (rc) return rc;
+  const dgc::MpnnNet net = net_of(m);
+  hipStream_t st = stream_of(m);
+  dgc::MpnnOutArgs a;
+  a.h3 = m->H[2];
+  a.t = T;
+  a.dy = m->dy;
+  a.dh3 = m->dh;
+  a.loss_partial = m->loss_partial;
+  a.inv_n = 1.0 / (double(m->K) * m->Np);
+  DGC_LAUNCH(dgc::mpnn_out(m->N, net, dgc::kOutTrain, a, st));
+  return backward_from(m, X);
+}
+
+// backward pass from m->dy (dL/dy) and m->dh (dL/dh3) to m->grad
+int backward_from(dgc_mpnn_s* m, const uint16_t* X) {
+  const dgc::MpnnNet net = net_of(m);
+  hipStream_t st = stream_of(m);
+  const dgc::MpnnLayout& L = m->L;
+  DGC_LAUNCH(dgc::mpnn_wgrad(net, m->Np, m->dy, dgc::kHid, m->H[2], false, m->partial, m->pstride, L.oWo, L.obo, st));
+  float* dh = m->dh;
+  float* dh2 = m->dh2;
+  for (int l = 2; l >= 0; --l) {
+    const uint16_t* hin = l == 0 ? X : m->H[l - 1];
+    const int din = L.din(l);
+    DGC_LAUNCH(dgc::mpnn_dense_bwd(net, l, m->H[l], dh, l > 0 ? m->u : nullptr, l > 0 ? m->vv : nullptr, st));
+    DGC_LAUNCH(dgc::mpnn_wgrad(net, dgc::kHid, dh, din, hin, false, m->partial, m->pstride, L.oA[l], L.ob[l], st));
+    DGC_LAUNCH(dgc::mpnn_wgrad(net, dgc::kHid, dh, din, hin, true, m->partial, m->pstride, L.oB[l], -1, st));
+    if (l > 0) {
+      DGC_LAUNCH(dgc::mpnn_nbr_accum(net, din, m->u, m->vv, dh2, st));
+      std::swap(dh, dh2);
+    }
+  }
+  DGC_LAUNCH(dgc::mpnn_reduce(m->partial, m->nblk, m->pstride, L.total, m->grad, st));
+  return DGC_OK;
+}
+
+int host_loss(dgc_mpnn_s* m, double* loss) {
+  std::vector<double> lp(m->out_grid);
+  DGC_HIP_TRY(hipMemcpyAsync(lp.data(), m->loss_partial, lp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             stream_of(m)));
+  DGC_HIP_TRY(hipStreamSynchronize(stream_of(m)));
+  *loss = std::accumulate(lp.begin(), lp.end(), 0.0) / (double(m->K) * m->Np);
+  return DGC_OK;
+}
+
+// zb = P r, partials (z.z, r.r) in L.partial
+int bj_apply(dgc_mpnn_s* m, const double* r, double* zb) {
+  IpdgLevel<double>& Lv = m->s->f64;
+  dgc::CgParams<double> cp{};
+  cp.K = m->K;
+  cp.x = Lv.x;
+  cp.r = const_cast<double*>(r);
+  cp.p = Lv.p;
+  cp.q = Lv.q;
+  cp.z = zb;
+  cp.P = Lv.P;
+  cp.scal = m->skip_scal;
+  cp.partial = Lv.partial;
+  cp.zz = 1;
+  DGC_LAUNCH(dgc::cg_update_launch<double>(m->N, cp, stream_of(m)));
+  return DGC_OK;
+}
+
+int precondition(dgc_mpnn_s* m, const double* r, const double* zb, const double* q, const double* cgscal);
+
+// Energy-norm step for one pair (r = S e, er = e.r): z = P_BJ r + s_z c_t NN,
+// w = S z, loss = (z.w - 2 z.r + er) / er; with `grad`, the parameter
+// gradient goes to m->grad. *loss (host) is filled if non-null.
+int energy_step(dgc_mpnn_s* m, const double* r, double er, bool grad, double* loss) {
+  hipStream_t st = stream_of(m);
+  IpdgLevel<double>& Lv = m->s->f64;
+  const double n = double(m->K) * m->Np;
+  int rc = bj_apply(m, r, Lv.z);
+  if (rc) return rc;
+  DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, n, nullptr, st));
+  if ((rc = precondition(m, r, Lv.z, nullptr, nullptr))) return rc;
+  dgc::IpdgParams<double> pr = Lv.params(*m->s);
+  pr.u = m->zfull;
+  pr.out = m->wbuf;
+  DGC_LAUNCH(dgc::ipdg_apply<double>(m->N, pr, dgc::kSpd, st));
+  DGC_LAUNCH(dgc::mpnn_energy_grad(m->N, net_of(m), m->zfull, m->wbuf, r, m->mscal, er, m->dy, m->dh,
+                                   m->partial_out, st));
+  if (grad && (rc = backward_from(m, m->X))) return rc;
+  if (loss) {
+    std::vector<double> pp(2 * size_t(m->out_grid));
+    DGC_HIP_TRY(hipMemcpyAsync(pp.data(), m->partial_out, pp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    double zw = 0.0, zr = 0.0;
+    for (int b = 0; b < m->out_grid; ++b) {
+      zw += pp[2 * b];
+      zr += pp[2 * b + 1];
+    }
+    *loss = (zw - 2.0 * zr + er) / er;
+  }
+  return DGC_OK;
+}
+
+// z = zb + s_z c_t NN(r, zb); partials (r.z, z.q) in m->partial_out
+int precondition(dgc_mpnn_s* m, const double* r, const double* zb, const double* q, const double* cgscal) {
+  const dgc::MpnnNet net = net_of(m);
+  hipStream_t st = stream_of(m);
+  DGC_LAUNCH(dgc::mpnn_features(m->N, net, r, zb, m->s->disc->geo, m->gnorm, m->mscal, m->X, st, cgscal));
+  int rc = forward(m, m->X, cgscal);
+  if (rc) return rc;
+  dgc::MpnnOutArgs a;
+  a.h3 = m->H[2];
+  a.zb = zb;
+  a.z = m->zfull;
+  a.r = r;
+  a.q = q;
+  a.partial = m->partial_out;
+  a.dscal = m->mscal;
+  DGC_LAUNCH(dgc::mpnn_out(m->N, net, dgc::kOutPrecond, a, st, cgscal));
+  return DGC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, dgc_mpnn_t* out) {
+  DGC_REQUIRE(s && out, "dgc_mpnn_create: null argument");
+  if (!s->f64.P) {
+    int rc = dgc_ipdg_block_jacobi(s, nullptr);
+    if (rc) return rc;
+  }
+  auto* m = new dgc_mpnn_s;
+  m->s = s;
+  m->K = s->disc->K;
+  m->N = s->disc->order;
+  m->Np = s->disc->Np;
+  m->L = dgc::mpnn_layout(m->Np);
+  m->nblk = dgc::mpnn_grid(m->K);
+  m->out_grid = (m->K + 255) / 256;
+  m->cg_grid = s->grid;
+  m->pstride = m->L.total;
+  const size_t K = m->K, P = m->L.total;
+  const int dmax = std::max(m->L.D0, dgc::kHid);
+  int rc = DGC_OK;
+  auto fail = [&](int code) {
+    for (void* p : m->owned) (void)hipFree(p);
+    delete m;
+    return code;
+  };
+  if ((rc = dalloc(m, &m->w, P)) || (rc = dalloc(m, &m->m, P)) || (rc = dalloc(m, &m->v, P)) ||
+      (rc = dalloc(m, &m->grad, P)) || (rc = dalloc(m, &m->wbf, P)) || (rc = dalloc(m, &m->partial, m->nblk * P)) ||
+      (rc = dalloc(m, &m->gnorm, 4)) || (rc = dalloc(m, &m->mscal, dgc::kMpnnScalars)) ||
+      (rc = dalloc(m, &m->skip_scal, dgc::kCgScalars)) || (rc = dalloc(m, &m->tmp_scal, dgc::kCgScalars)) ||
+      (rc = dalloc(m, &m->X, K * m->L.D0)) ||
+      (rc = dalloc(m, &m->dh, K * dgc::kHid)) || (rc = dalloc(m, &m->dh2, K * dgc::kHid)) ||
+      (rc = dalloc(m, &m->u, K * dmax)) || (rc = dalloc(m, &m->vv, K * dmax)) || (rc = dalloc(m, &m->dy, K * m->Np)) ||
+      (rc = dalloc(m, &m->y, K * m->Np)) || (rc = dalloc(m, &m->tgt, K * m->Np)) ||
+      (rc = dalloc(m, &m->loss_partial, m->out_grid)) || (rc = dalloc(m, &m->partial_out, 2 * m->out_grid)) ||
+      (rc = dalloc(m, &m->zfull, K * m->Np)) || (rc = dalloc(m, &m->amp, 32 * 32)) ||
+      (rc = dalloc(m, &m->wbuf, K * m->Np)) || (rc = dalloc(m, &m->dotpart, 257)))
+    return fail(rc);
+  for (int l = 0; l < 3; ++l) {
+    if ((rc = dalloc(m, &m->H[l], K * dgc::kHid))) return fail(rc);
+    if ((rc = dalloc(m, &m->wpk[l], size_t(m->L.din(l)) / 4 * 2 * 64))) return fail(rc);
+  }
+  // feature normalisation of the metric terms: their RMS over the cells
+  std::vector<double> geo(4 * K);
+  if (hipMemcpy(geo.data(), s->disc->geo, geo.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(dgc_set_error(DGC_ERR_HIP, "dgc_mpnn_create: geometry readback failed"));
+  float gn[4];
+  for (int k = 0; k < 4; ++k) {
+    double a = 0.0;
+    for (size_t c = 0; c < K; ++c) a += geo[k * K + c] * geo[k * K + c];
+    gn[k] = float(std::sqrt(a / K));
+    if (!(gn[k] > 0.f)) gn[k] = 1.f;
+  }
+  double sk[dgc::kCgScalars] = {0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
+  double ms[dgc::kMpnnScalars] = {1.0, 1.0, 1.0, 0.0};
+  if (hipMemcpy(m->gnorm, gn, sizeof(gn), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->skip_scal, sk, sizeof(sk), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->mscal, ms, sizeof(ms), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(dgc_set_error(DGC_ERR_HIP, "dgc_mpnn_create: upload failed"));
+  // He initialisation, deterministic
+  std::vector<float> w(P, 0.f);
+  Gauss g(seed);
+  const dgc::MpnnLayout& L = m->L;
+  for (int l = 0; l < 3; ++l) {
+    const int d = L.din(l);
+    const double sd = std::sqrt(2.0 / (2.0 * d));
+    for (int i = 0; i < dgc::kHid * d; ++i) w[L.oA[l] + i] = float(sd * g());
+    for (int i = 0; i < dgc::kHid * d; ++i) w[L.oB[l] + i] = float(sd * g());
+  }
+  // the output layer starts at zero: the untrained network is block-Jacobi
+  if ((rc = upload_weights(m, w))) return fail(rc);
+  *out = m;
+  return DGC_OK;
+}
+
+int dgc_mpnn_destroy(dgc_mpnn_t m) {
+  if (!m) return DGC_OK;
+  (void)hipStreamSynchronize(stream_of(m));
+  if (m->graph) (void)hipGraphExecDestroy(m->graph);
+  for (void* p : m->owned) (void)hipFree(p);
+  (void)hipFree(m->ds_r);
+  (void)hipFree(m->dof_x);
+  (void)hipFree(m->dof_y);
+  (void)hipFree(m->e);
+  delete m;
+  return DGC_OK;
+}
+
+int dgc_mpnn_param_count(dgc_mpnn_t m) { return m ? m->L.total : 0; }
+int dgc_mpnn_input_width(dgc_mpnn_t m) { return m ? m->L.D0 : 0; }
+
+int dgc_mpnn_get_params(dgc_mpnn_t m, float* w) {
+  DGC_REQUIRE(m && w, "dgc_mpnn_get_params: null argument");
+  DGC_HIP_TRY(hipMemcpy(w, m->w, m->L.total * sizeof(float), hipMemcpyDeviceToHost));
+  return DGC_OK;
+}
+
+int dgc_mpnn_set_params(dgc_mpnn_t m, const float* w) {
+  DGC_REQUIRE(m && w, "dgc_mpnn_set_params: null argument");
+  for (int i = 0; i < m->L.total; ++i) DGC_REQUIRE(std::isfinite(w[i]), "dgc_mpnn_set_params: non-finite weight");
+  return upload_weights(m, std::vector<float>(w, w + m->L.total));
+}
+
+int dgc_mpnn_eval(dgc_mpnn_t m, const float* X, float* y) {
+  DGC_REQUIRE(m && X && y, "dgc_mpnn_eval: null argument");
+  const size_t K = m->K, D0 = m->L.D0;
+  std::vector<uint16_t> xb(K * D0);
+  for (size_t i = 0; i < xb.size(); ++i) xb[i] = dgc::f2bf(X[i]);
+  hipStream_t st = stream_of(m);
+  DGC_HIP_TRY(hipMemcpyAsync(m->X, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, st));
+  int rc = forward(m, m->X);
+  if (rc) return rc;
+  dgc::MpnnOutArgs a;
+  a.h3 = m->H[2];
+  a.y = m->y;
+  DGC_LAUNCH(dgc::mpnn_out(m->N, net_of(m), dgc::kOutY, a, st));
+  DGC_HIP_TRY(hipMemcpyAsync(y, m->y, K * m->Np * sizeof(float), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
+int dgc_mpnn_loss_grad(dgc_mpnn_t m, const float* X, const float* T, double* loss, float* grad) {
+  DGC_REQUIRE(m && X && T, "dgc_mpnn_loss_grad: null argument");
+  const size_t K = m->K, D0 = m->L.D0;
+  std::vector<uint16_t> xb(K * D0);
+  for (size_t i = 0; i < xb.size(); ++i) xb[i] = dgc::f2bf(X[i]);
+  hipStream_t st = stream_of(m);
+  DGC_HIP_TRY(hipMemcpyAsync(m->X, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipMemcpyAsync(m->tgt, T, K * m->Np * sizeof(float), hipMemcpyHostToDevice, st));
+  int rc = loss_grad(m, m->X, m->tgt);
+  if (rc) return rc;
+  if (grad) DGC_HIP_TRY(hipMemcpyAsync(grad, m->grad, m->L.total * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (loss) {
+    rc = host_loss(m, loss);
+    if (rc) return rc;
+  }
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
+int dgc_mpnn_target_scale(dgc_mpnn_t m, double* ct) {
+  DGC_REQUIRE(m && ct, "dgc_mpnn_target_scale: null argument");
+  *ct = m->ct;
+  return DGC_OK;
+}
+
+int dgc_mpnn_set_target_scale(dgc_mpnn_t m, double ct) {
+  DGC_REQUIRE(m && std::isfinite(ct) && ct > 0, "dgc_mpnn_set_target_scale: need a finite ct > 0");
+  hipStream_t st = stream_of(m);
+  m->ct = ct;
+  DGC_HIP_TRY(hipMemcpyAsync(m->mscal + dgc::kCt, &m->ct, sizeof(double), hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
+int dgc_mpnn_energy_loss_grad(dgc_mpnn_t m, const double* r, double er, double* loss, float* grad, double* z) {
+  DGC_REQUIRE(m && r, "dgc_mpnn_energy_loss_grad: null argument");
+  DGC_REQUIRE(std::isfinite(er) && er > 0, "dgc_mpnn_energy_loss_grad: need e.r > 0");
+  hipStream_t st = stream_of(m);
+  int rc = energy_step(m, r, er, grad != nullptr, loss);
+  if (rc) return rc;
+  if (grad) DGC_HIP_TRY(hipMemcpyAsync(grad, m->grad, m->L.total * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (z)
+    DGC_HIP_TRY(hipMemcpyAsync(z, m->zfull, size_t(m->K) * m->Np * sizeof(double), hipMemcpyDeviceToDevice, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
+int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nsamples, uint64_t seed0,
+                           int modes) {
+  DGC_REQUIRE(m && dof_x && dof_y, "dgc_mpnn_build_dataset: null argument");
+  DGC_REQUIRE(nsamples > 0 && modes >= 1 && modes <= 32, "dgc_mpnn_build_dataset: bad nsamples or modes");
+  hipStream_t st = stream_of(m);
+  const size_t n = size_t(m->K) * m->Np;
+  if (!m->dof_x) {
+    DGC_HIP_TRY(hipMalloc(&m->dof_x, n * sizeof(double)));
+    DGC_HIP_TRY(hipMalloc(&m->dof_y, n * sizeof(double)));
+    DGC_HIP_TRY(hipMalloc(&m->e, n * sizeof(double)));
+  }
+  DGC_HIP_TRY(hipMemcpy(m->dof_x, dof_x, n * sizeof(double), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(m->dof_y, dof_y, n * sizeof(double), hipMemcpyHostToDevice));
+  (void)hipFree(m->ds_r);
+  m->ds_r = nullptr;
+  m->nsamples = 0;
+  DGC_HIP_TRY(hipMalloc(&m->ds_r, size_t(nsamples) * n * sizeof(double)));
+  m->ds_er.assign(nsamples, 0.0);
+  IpdgLevel<double>& Lv = m->s->f64;
+  double* zb = Lv.z;
+  std::vector<double> amp(size_t(modes) * modes);
+  for (int smp = 0; smp < nsamples; ++smp) {
+    // error field: sine modes of [-1,1]^2 with N(0,1) k^-2 amplitudes
+    Gauss g(seed0 + uint64_t(smp));
+    for (int a = 1; a <= modes; ++a)
+      for (int b = 1; b <= modes; ++b) amp[size_t(a - 1) * modes + b - 1] = g() / double(a * a + b * b);
+    DGC_HIP_TRY(hipMemcpyAsync(m->amp, amp.data(), amp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    DGC_LAUNCH(dgc::grf_eval(m->dof_x, m->dof_y, m->amp, modes, n, m->e, st));
+    // residual r = S e through the solver's operator; e.r = |e|_S^2
+    double* r = m->ds_r + size_t(smp) * n;
+    dgc::IpdgParams<double> pr = Lv.params(*m->s);
+    pr.u = m->e;
+    pr.out = r;
+    DGC_LAUNCH(dgc::ipdg_apply<double>(m->N, pr, dgc::kSpd, st));
+    DGC_LAUNCH(dgc::dot_device(m->e, r, n, m->dotpart, m->dotpart + 256, st));
+    DGC_HIP_TRY(hipMemcpyAsync(&m->ds_er[smp], m->dotpart + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (smp == 0) {
+      // zb = P_BJ r and the input scales for the target-scale estimate
+      int rc = bj_apply(m, r, zb);
+      if (rc) return rc;
+      DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, double(n), nullptr, st));
+      // target scale c_t: RMS of (e - zb) / s_z on the first sample
+      std::vector<double> he(n), hz(n), ms(dgc::kMpnnScalars);
+      DGC_HIP_TRY(hipMemcpyAsync(he.data(), m->e, n * sizeof(double), hipMemcpyDeviceToHost, st));
+      DGC_HIP_TRY(hipMemcpyAsync(hz.data(), zb, n * sizeof(double), hipMemcpyDeviceToHost, st));
+      DGC_HIP_TRY(hipMemcpyAsync(ms.data(), m->mscal, ms.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      DGC_HIP_TRY(hipStreamSynchronize(st));
+      double acc = 0.0;
+      for (size_t i = 0; i < n; ++i) {
+        const double d = (he[i] - hz[i]) / ms[dgc::kSz];
+        acc += d * d;
+      }
+      m->ct = std::sqrt(acc / double(n));
+      DGC_REQUIRE(std::isfinite(m->ct) && m->ct > 0, "dgc_mpnn_build_dataset: degenerate target scale");
+      ms[dgc::kCt] = m->ct;
+      DGC_HIP_TRY(hipMemcpyAsync(m->mscal, ms.data(), ms.size() * sizeof(double), hipMemcpyHostToDevice, st));
+      DGC_HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  for (int smp = 0; smp < nsamples; ++smp)
+    DGC_REQUIRE(std::isfinite(m->ds_er[smp]) && m->ds_er[smp] > 0, "dgc_mpnn_build_dataset: e.r is not positive");
+  m->nsamples = nsamples;
+  return DGC_OK;
+}
+
+int dgc_mpnn_train(dgc_mpnn_t m, int first, int count, int epochs, float lr, uint64_t shuffle_seed,
+                   double* epoch_loss) {
+  DGC_REQUIRE(m, "dgc_mpnn_train: null network");
+  DGC_REQUIRE(m->nsamples > 0, "dgc_mpnn_train: build a dataset first");
+  DGC_REQUIRE(first >= 0 && count > 0 && first + count <= m->nsamples, "dgc_mpnn_train: sample range");
+  DGC_REQUIRE(epochs > 0 && lr > 0.f, "dgc_mpnn_train: bad epochs or learning rate");
+  hipStream_t st = stream_of(m);
+  const size_t n = size_t(m->K) * m->Np;
+  std::vector<int> order(count);
+  dgc_host::SplitMix64 rng(shuffle_seed);
+  const int total_steps = epochs * count;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::iota(order.begin(), order.end(), first);
+    for (int i = count - 1; i > 0; --i) std::swap(order[i], order[int(rng.next() % uint64_t(i + 1))]);
+    double acc = 0.0;
+    for (int k = 0; k < count; ++k) {
+      const int smp = order[k];
+      double l = 0.0;
+      int rc = energy_step(m, m->ds_r + size_t(smp) * n, m->ds_er[smp], true, &l);
+      if (rc) return rc;
+      if (!std::isfinite(l)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_mpnn_train: loss is not finite");
+      acc += l;
+      const int step = ep * count + k;
+      // cosine decay to a tenth of the initial rate
+      const float lr_t = lr * float(0.1 + 0.9 * 0.5 * (1.0 + std::cos(M_PI * step / double(total_steps))));
+      DGC_LAUNCH(dgc::mpnn_adam(m->w, m->m, m->v, m->grad, m->L.total, lr_t, 0.9f, 0.999f, 1e-8f, ++m->adam_step,
+                                m->wbf, st));
+      DGC_LAUNCH(dgc::mpnn_pack(m->L, m->wbf, m->wpk, st));
+    }
+    if (epoch_loss) epoch_loss[ep] = acc / count;
+  }
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
+int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss) {
+  DGC_REQUIRE(m && loss, "dgc_mpnn_eval_loss: null argument");
+  DGC_REQUIRE(first >= 0 && count > 0 && first + count <= m->nsamples, "dgc_mpnn_eval_loss: sample range");
+  const size_t n = size_t(m->K) * m->Np;
+  double acc = 0.0;
+  for (int smp = first; smp < first + count; ++smp) {
+    double l = 0.0;
+    int rc = energy_step(m, m->ds_r + size_t(smp) * n, m->ds_er[smp], false, &l);
+    if (rc) return rc;
+    acc += l;
+  }
+  *loss = acc / count;
+  return DGC_OK;
+}
+
+// Flexible CG (one-term recurrence, p_{k+1} = z_{k+1} - (z_{k+1}.q_k)/(p_k.q_k) p_k)
+// with z = P_BJ r + s_z c_t NN(r, P_BJ r). Runs in graph-captured chunks; falls
+// back to block-Jacobi PCG from the current iterate if the residual stalls.
+int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double rtol, int maxit, int stall_window,
+                 double stall_factor, int* iters, double* relres, int* fell_back, double* history, float* ms) {
+  DGC_REQUIRE(s && m && b && x && m->s == s, "dgc_ipdg_fcg: bad argument");
+  DGC_REQUIRE(rtol > 0 && maxit > 0, "dgc_ipdg_fcg: need rtol > 0 and maxit > 0");
+  DGC_REQUIRE(m->ct > 0, "dgc_ipdg_fcg: the network has no target scale (build a dataset first)");
+  dgc_disc_s* d = s->disc;
+  hipStream_t st = d->ctx->stream;
+  const size_t n = size_t(d->K) * d->Np;
+  IpdgLevel<double>& Lv = s->f64;
+  Lv.use_P = true;
+  if (s->hist_len < maxit + 1) {
+    (void)hipFree(s->hist);
+    s->hist = nullptr;
+    DGC_HIP_TRY(hipMalloc(&s->hist, (size_t(maxit) + 1) * sizeof(double)));
+    s->hist_len = maxit + 1;
+    if (Lv.graph) (void)hipGraphExecDestroy(Lv.graph);
+    Lv.graph = nullptr;
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+  }
+  std::vector<double> hb(n);
+  DGC_HIP_TRY(hipMemcpyAsync(hb.data(), b, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  double bb = 0.0;
+  for (double v : hb) bb += v * v;
+  DGC_REQUIRE(std::isfinite(bb) && bb > 0, "dgc_ipdg_fcg: zero or non-finite right-hand side");
+  const double nd = double(n);
+
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventRecord(e0, st));
+  // ---- initialisation: r = b, x = 0, z = M(r), p = z
+  double h[dgc::kCgScalars] = {0, 0, 0, 0, 0, 0, rtol * rtol * bb, double(maxit), 0, 0};
+  DGC_HIP_TRY(hipMemcpyAsync(Lv.scal, h, sizeof(h), hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipMemsetAsync(Lv.x, 0, n * sizeof(double), st));
+  DGC_HIP_TRY(hipMemsetAsync(Lv.p, 0, n * sizeof(double), st));
+  DGC_HIP_TRY(hipMemcpyAsync(Lv.r, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int rc = bj_apply(m, Lv.r, Lv.z);
+  if (rc) return rc;
+  DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, Lv.scal, m->mscal, nd, s->hist, st));
+  if ((rc = precondition(m, Lv.r, Lv.z, nullptr, Lv.scal))) return rc;
+  DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->out_grid, 2, Lv.scal, m->mscal, nd, nullptr, st));
+  DGC_LAUNCH(dgc::cg_direction_launch<double>(m->zfull, Lv.p, Lv.scal, n, st));
+
+  // ---- one iteration, captured in chunks
+  auto iteration = [&]() -> int {
+    dgc::IpdgParams<double> pr = Lv.params(*s);
+    pr.u = Lv.p;
+    pr.out = Lv.q;
+    pr.dotv = Lv.p;
+    pr.partial = Lv.partial;
+    pr.scal = Lv.scal;
+    DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
+    DGC_LAUNCH(dgc::cg_scalars_launch<double>(Lv.partial, s->grid, dgc::kAfterApply, Lv.scal, nullptr, st));
+    dgc::CgParams<double> cp{};
+    cp.K = d->K;
+    cp.x = Lv.x;
+    cp.r = Lv.r;
+    cp.p = Lv.p;
+    cp.q = Lv.q;
+    cp.z = Lv.z;
+    cp.P = Lv.P;
+    cp.scal = Lv.scal;
+    cp.partial = Lv.partial;
+    cp.zz = 1;
+    DGC_LAUNCH(dgc::cg_update_launch<double>(d->order, cp, st));
+    DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 0, Lv.scal, m->mscal, nd, s->hist, st));
+    int r2 = precondition(m, Lv.r, Lv.z, Lv.q, Lv.scal);
+    if (r2) return r2;
+    DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->out_grid, 1, Lv.scal, m->mscal, nd, nullptr, st));
+    DGC_LAUNCH(dgc::cg_direction_launch<double>(m->zfull, Lv.p, Lv.scal, n, st));
+    return DGC_OK;
+  };
+  const int chunk = 8;
+  if (!m->graph || m->graph_hist != s->hist) {
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+    hipGraph_t g = nullptr;
+    DGC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk && rc == DGC_OK; ++i) rc = iteration();
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    if (rc != DGC_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ce != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("fcg capture: ") + hipGetErrorString(ce));
+    ce = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ce != hipSuccess) {
+      m->graph = nullptr;
+      return dgc_set_error(DGC_ERR_HIP, std::string("fcg instantiate: ") + hipGetErrorString(ce));
+    }
+    m->graph_hist = s->hist;
+  }
+  std::vector<double> hist_h;
+  int it = 0, fb = 0;
+  for (;;) {
+    DGC_HIP_TRY(hipMemcpyAsync(h, Lv.scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    it = int(h[dgc::kIter]);
+    if (h[dgc::kDone] != 0.0 || it >= maxit) break;
+    if (stall_window > 0 && it >= stall_window) {
+      hist_h.resize(it + 1);
+      DGC_HIP_TRY(hipMemcpy(hist_h.data(), s->hist, (it + 1) * sizeof(double), hipMemcpyDeviceToHost));
+      const double now = hist_h[it], before = hist_h[it - stall_window];
+      if (!(now == now) || now > stall_factor * stall_factor * before) {
+        fb = 1;
+        break;
+      }
+    }
+    DGC_HIP_TRY(hipGraphLaunch(m->graph, st));
+  }
+  const bool nan_state = !(h[dgc::kRr] == h[dgc::kRr]);
+  if (nan_state) {
+    // the network produced non-finite values: restart from zero with block-Jacobi PCG
+    DGC_HIP_TRY(hipMemsetAsync(Lv.x, 0, n * sizeof(double), st));
+    fb = 1;
+  }
+  // true FP64 residual; fall back to block-Jacobi PCG from x if needed
+  auto true_rel = [&](double& rel) -> int {
+    dgc::IpdgParams<double> pr = Lv.params(*s);
+    pr.u = Lv.x;
+    pr.out = Lv.q;
+    DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
+    std::vector<double> hq(n), hx(n);
+    DGC_HIP_TRY(hipMemcpyAsync(hq.data(), Lv.q, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
+    double rr = 0.0;
+    for (size_t i = 0; i < n; ++i) rr += (hb[i] - hq[i]) * (hb[i] - hq[i]);
+    rel = std::sqrt(rr / bb);
+    return DGC_OK;
+  };
+  double rel = 1.0;
+  if ((rc = true_rel(rel))) return rc;
+  if ((fb || rel > rtol) && it < maxit) {
+    if (!fb) fb = 2;  // finished by a PCG restart after recurrence drift
+    rc = cg_init_fp64_restart(s, b, rtol * rtol * bb, maxit - it, s->hist + it);
+    int it2 = 0;
+    if (rc == DGC_OK) rc = cg_run_fp64(s, s->hist + it, maxit - it, &it2);
+    if (rc) return rc;
+    it += it2;
+    if ((rc = true_rel(rel))) return rc;
+  }
+  DGC_HIP_TRY(hipEventRecord(e1, st));
+  DGC_HIP_TRY(hipEventSynchronize(e1));
+  float el = 0.f;
+  (void)hipEventElapsedTime(&el, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  DGC_HIP_TRY(hipMemcpyAsync(x, Lv.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (history) DGC_HIP_TRY(hipMemcpyAsync(history, s->hist, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  if (fell_back) *fell_back = fb;
+  if (ms) *ms = el;
+  if (rel > rtol) return dgc_set_error(DGC_ERR_NOT_CONVERGED, "dgc_ipdg_fcg: iteration cap reached");
+  return DGC_OK;
+}
+
+}  // extern "C"

@@ -19,3 +19,4 @@ from .core import (  # noqa: F401
     cell_to_node_major,
 )
 from .ipdg import IPDG, poisson_rhs  # noqa: F401
+from .mpnn import LearnedPreconditioner  # noqa: F401

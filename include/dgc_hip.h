@@ -198,6 +198,54 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
                        int max_outer, int max_inner, int* outer_iters, int* inner_iters,
                        double* relres, float* ms);
 
+/* ---- learned preconditioner (no reference counterpart) --------------------
+ * A 3-layer message-passing network over the cells' face adjacency (hidden
+ * width 64, bf16 weights/activations, fp32 accumulation; matrix-core forward
+ * layers) maps (r, P_BJ r, cell metrics) to a correction of the block-Jacobi
+ * step: z = P_BJ r + s_z c_t NN(r / s_r, P_BJ r / s_z, g). It is trained on
+ * the GPU on residuals r = S e made with the solver's own operator from
+ * seeded random error fields e, with the energy-norm loss
+ * |z - e|_S^2 / |e|_S^2 (the error CG minimises), and used inside flexible CG. */
+typedef struct dgc_mpnn_s* dgc_mpnn_t;
+int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, dgc_mpnn_t* out);
+int dgc_mpnn_destroy(dgc_mpnn_t m);
+int dgc_mpnn_param_count(dgc_mpnn_t m);
+/* width D0 of the input feature vector per cell (2 Np + 4 rounded up to 8) */
+int dgc_mpnn_input_width(dgc_mpnn_t m);
+/* parameters as one fp32 vector: A1 B1 b1 A2 B2 b2 A3 B3 b3 Wo bo (row-major) */
+int dgc_mpnn_get_params(dgc_mpnn_t m, float* w);
+int dgc_mpnn_set_params(dgc_mpnn_t m, const float* w);
+/* y [K][Np] = network(X [K][D0]) (X rounded to bf16 on upload) */
+int dgc_mpnn_eval(dgc_mpnn_t m, const float* X, float* y);
+/* mean-square loss against T [K][Np] and its gradient [param_count] */
+int dgc_mpnn_loss_grad(dgc_mpnn_t m, const float* X, const float* T, double* loss, float* grad);
+/* Training pairs: for sample i the error e is a sum of modes^2 Dirichlet sine
+ * modes of [-1,1]^2 with N(0,1) (m^2+n^2)^-1 amplitudes (seed seed0 + i),
+ * evaluated at the DOF coordinates dof_x/dof_y (host, node-major), r = S e.
+ * Also sets the target scale c_t = RMS((e - P_BJ r) / s_z) of sample 0. */
+int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nsamples,
+                           uint64_t seed0, int modes);
+int dgc_mpnn_target_scale(dgc_mpnn_t m, double* ct);
+int dgc_mpnn_set_target_scale(dgc_mpnn_t m, double ct);
+/* Energy-norm loss of one residual r = S e (device, node-major) with
+ * er = e.r: z = P_BJ r + s_z c_t y, loss = (z.Sz - 2 z.r + er) / er, its
+ * gradient [param_count] (host, may be null) and z (device, may be null). */
+int dgc_mpnn_energy_loss_grad(dgc_mpnn_t m, const double* r, double er, double* loss, float* grad, double* z);
+/* Adam with cosine decay over samples [first, first+count), one sample per
+ * step on the energy-norm loss; epoch_loss[epochs] receives the mean training
+ * loss of each epoch. */
+int dgc_mpnn_train(dgc_mpnn_t m, int first, int count, int epochs, float lr, uint64_t shuffle_seed,
+                   double* epoch_loss);
+int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss);
+/* Flexible CG on (-M Lap_h) x = b with the network preconditioner, to the FP64
+ * true relative residual rtol. If the squared residual fails to drop below
+ * stall_factor^2 times its value stall_window iterations earlier (or turns
+ * non-finite), the solve continues with block-Jacobi PCG from the current
+ * iterate (*fell_back = 1; 2 = finished by a PCG restart after drift). */
+int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double rtol, int maxit,
+                 int stall_window, double stall_factor, int* iters, double* relres, int* fell_back,
+                 double* history, float* ms);
+
 /* Device self-test of the matrix-core operand layouts the kernels rely on:
  * kind 0: C[32x32] = A[32x8] B[8x32] with bf16 inputs (rounded on the device),
  * kind 1: C[16x16] = A[16x4] B[4x16] in fp32. Row-major host arrays; the

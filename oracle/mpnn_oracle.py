@@ -71,10 +71,30 @@ def forward(p, x, nbr, bf=False):
 
 
 def loss_and_grads(p, x, t, nbr, bf=False):
+    """Mean-square loss against targets t and its parameter gradients."""
     y, c = forward(p, x, nbr, bf)
     n = y.size
     loss = float(np.mean((y - t) ** 2))
-    dy = 2.0 * (y - t) / n
+    g = backward(p, c, 2.0 * (y - t) / n, nbr)
+    return loss, g, y
+
+
+def energy_loss_and_grads(p, x, nbr, zb, r, e, scale, apply_S, bf=False):
+    """Energy-norm loss of the preconditioned vector z = zb + scale * y:
+        L = (z - e)^T S (z - e) / (e^T S e),  S e = r,
+    with dL/dy = scale * 2 (S z - r) / (e^T r). zb, r, e: (K, Np); apply_S maps
+    a (K, Np) array to S times it."""
+    y, c = forward(p, x, nbr, bf)
+    z = zb + scale * y
+    Sz = apply_S(z)
+    er = float(np.sum(e * r))
+    loss = float((np.sum(z * Sz) - 2.0 * np.sum(z * r) + er) / er)
+    g = backward(p, c, scale * 2.0 * (Sz - r) / er, nbr)
+    return loss, g, y
+
+
+def backward(p, c, dy, nbr):
+    """Parameter gradients for the output gradient dy (K, Np)."""
     q = c["q"]
     g = {"Wo": dy.T @ c["h3"], "bo": dy.sum(0)}
     dh = dy @ q["Wo"]
@@ -85,7 +105,7 @@ def loss_and_grads(p, x, t, nbr, bf=False):
         g[f"b{l}"] = dz.sum(0)
         if l > 1:
             dh = dz @ q[f"A{l}"] + neighbour_sum(dz @ q[f"B{l}"], nbr)
-    return loss, g, y
+    return g
 
 
 def features(r, zb, metric, Np):

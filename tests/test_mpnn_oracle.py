@@ -67,3 +67,41 @@ def test_bf16_rounding_helper():
     assert r[0] == 1.0 and r[1] == 1.0  # halfway rounds to even
     assert r[2] == 1.0 + 2**-7
     assert abs(r[3] + 3.140625) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["A1", "B2", "b3", "Wo", "bo"])
+def test_energy_loss_gradients_match_finite_differences(name):
+    p, x, t, nbr = _tiny(seed=2)
+    K, Np = t.shape
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((K * Np, K * Np))
+    S = G @ G.T + K * Np * np.eye(K * Np)  # SPD stand-in for the operator
+
+    def apply_S(v):
+        return (S @ v.reshape(-1)).reshape(K, Np)
+
+    e = rng.standard_normal((K, Np))
+    r = apply_S(e)
+    zb = 0.1 * rng.standard_normal((K, Np))
+    _, g, _ = mo.energy_loss_and_grads(p, x, nbr, zb, r, e, 0.7, apply_S)
+    flat = p[name].reshape(-1)
+    for idx in rng.choice(flat.size, size=min(4, flat.size), replace=False):
+        h = 1e-6
+        old = flat[idx]
+        flat[idx] = old + h
+        lp, _, _ = mo.energy_loss_and_grads(p, x, nbr, zb, r, e, 0.7, apply_S)
+        flat[idx] = old - h
+        lm, _, _ = mo.energy_loss_and_grads(p, x, nbr, zb, r, e, 0.7, apply_S)
+        flat[idx] = old
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[name].reshape(-1)[idx]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_energy_loss_is_zero_at_the_exact_error():
+    p, x, t, nbr = _tiny(seed=4)
+    K, Np = t.shape
+    for k in p:
+        p[k] = np.zeros_like(p[k])
+    e = np.random.default_rng(0).standard_normal((K, Np))
+    loss, _, _ = mo.energy_loss_and_grads(p, x, nbr, e, 2.0 * e, e, 1.0, lambda v: 2.0 * v)
+    assert abs(loss) < 1e-14
