@@ -62,4 +62,35 @@ template <typename T>
 hipError_t cg_scalars_launch(const T* partial, int nparts, int which, T* scal, double* history,
                              hipStream_t s);
 
+// ---- coarse space (dg_coarse.hip) -------------------------------------------
+// Three linear functions per aggregate of cells, phi_0 = 1,
+// phi_1 = (x - x_a)/h_a, phi_2 = (y - y_a)/h_a. On cell c they are affine in the
+// reference coordinates (rho, sigma) of its nodes:
+//   phi_k(node j) = coef[3k-3][c] + coef[3k-2][c] rho_j + coef[3k-1][c] sigma_j  (k = 1, 2).
+struct CoarseSpace {
+  int K = 0, na = 0, nc = 0, ld = 0;  // cells, aggregates, unknowns (3 na), row stride of Ainv
+  const int* agg_ptr = nullptr;       // [na + 1] CSR over the cells of each aggregate
+  const int* agg_cells = nullptr;     // [K]
+  const int* cell_agg = nullptr;      // [K]
+  const float* coef = nullptr;        // [6][K]
+  const double* rs = nullptr;         // [2][Np] reference node coordinates (rho, sigma)
+  const float* Ainv = nullptr;        // [nc][ld] inverse of Phi^T S Phi (fp32, symmetric)
+  double* rc = nullptr;               // [nc] restricted vector
+  double* yc = nullptr;               // [nc] coarse solution
+};
+// out[(3a + k) * ostride] = sum over the aggregate's nodes of phi_k v (wave per aggregate)
+hipError_t coarse_restrict(int order, const CoarseSpace& cs, const double* v, double* out, int ostride,
+                           const double* scal, hipStream_t s);
+// yc = Ainv rc
+hipError_t coarse_solve(const CoarseSpace& cs, const double* scal, hipStream_t s);
+// z += Phi yc; partials (r.z, r.r), or (z.z, r.r) when zz != 0, per tile of 64 cells
+hipError_t coarse_prolong(int order, const CoarseSpace& cs, const double* r, double* z, double* partial, int zz,
+                          const double* scal, hipStream_t s);
+// u = column `col` of Phi (u zeroed by the caller)
+hipError_t coarse_fill(int order, const CoarseSpace& cs, int col, double* u, hipStream_t s);
+// In-place Gauss-Jordan inverse of an SPD n x n matrix (no pivoting); work holds 2n doubles.
+hipError_t spd_inverse_inplace(double* A, int n, double* work, hipStream_t s);
+// Ainv[i][j] = float((A[i][j] + A[j][i]) / 2), row stride ld, zero padding
+hipError_t symmetrize_to_float(const double* A, int n, int ld, float* Ainv, hipStream_t s);
+
 }  // namespace dgc

@@ -214,6 +214,13 @@ static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_t
   cp.scal = L.scal;
   cp.partial = L.partial;
   DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
+  if constexpr (std::is_same<T, double>::value) {
+    // the done flag is set while initialising: run the correction unconditionally
+    if (L.use_coarse) {
+      int rc = coarse_correct(s, L.r, L.z, L.partial, 0, nullptr);
+      if (rc) return rc;
+    }
+  }
   DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kInit, L.scal, history, st));
   DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
   return DGC_OK;
@@ -243,6 +250,12 @@ static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
   cp.scal = L.scal;
   cp.partial = L.partial;
   DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
+  if constexpr (std::is_same<T, double>::value) {
+    if (L.use_coarse) {
+      int rc = coarse_correct(s, L.r, L.z, L.partial, 0, L.scal);
+      if (rc) return rc;
+    }
+  }
   DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterUpdate, L.scal, history, st));
   DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
   return DGC_OK;
@@ -253,7 +266,7 @@ template <typename T>
 static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, int* iters) {
   hipStream_t st = s->disc->ctx->stream;
   const int chunk = 16;
-  if (!L.graph || L.graph_hist != history || L.graph_P != L.use_P) {
+  if (!L.graph || L.graph_hist != history || L.graph_P != L.use_P || L.graph_coarse != L.use_coarse) {
     if (L.graph) (void)hipGraphExecDestroy(L.graph);
     L.graph = nullptr;
     hipGraph_t g = nullptr;
@@ -274,6 +287,7 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
     }
     L.graph_hist = history;
     L.graph_P = L.use_P;
+    L.graph_coarse = L.use_coarse;
   }
   T h[dgc::kCgScalars];
   for (int done_chunks = 0;; ++done_chunks) {
@@ -295,6 +309,8 @@ int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, i
 int cg_run_fp64(dgc_ipdg_s* s, double* history, int maxit, int* iters) {
   return cg_run<double>(s, s->f64, history, maxit, iters);
 }
+
+static void release_coarse(dgc_ipdg_s* s);
 
 static void face_penalty(const dgc_disc_s* d, int tau_mode, double tau_scale, std::vector<double>& tau) {
   const int K = d->K, N = d->order;
@@ -352,6 +368,7 @@ int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const d
 int dgc_ipdg_destroy(dgc_ipdg_t s) {
   if (!s) return DGC_OK;
   (void)hipStreamSynchronize(s->disc->ctx->stream);
+  release_coarse(s);
   s->f64.release();
   s->f32.release();
   (void)hipFree(s->hist);
@@ -486,12 +503,200 @@ int dgc_ipdg_diag_blocks(dgc_ipdg_t s, double* blocks) {
   return DGC_OK;
 }
 
+}  // extern "C"
+
+static void release_coarse(dgc_ipdg_s* s) {
+  // captured solver graphs hold the coarse arrays' addresses
+  if (s->f64.graph) (void)hipGraphExecDestroy(s->f64.graph);
+  s->f64.graph = nullptr;
+  ++s->coarse_gen;
+  for (void* p : s->coarse_owned) (void)hipFree(p);
+  s->coarse_owned.clear();
+  s->coarse = dgc::CoarseSpace{};
+  s->coarse_A.clear();
+  s->f64.use_coarse = false;
+}
+
+template <typename T>
+static int coarse_alloc(dgc_ipdg_s* s, T** p, size_t n) {
+  int rc = alloc(p, n);
+  if (rc == DGC_OK) s->coarse_owned.push_back(*p);
+  return rc;
+}
+
+int coarse_correct(dgc_ipdg_s* s, const double* r, double* z, double* partial, int zz, const double* scal) {
+  hipStream_t st = s->disc->ctx->stream;
+  const int N = s->disc->order;
+  DGC_LAUNCH(dgc::coarse_restrict(N, s->coarse, r, s->coarse.rc, 1, scal, st));
+  DGC_LAUNCH(dgc::coarse_solve(s->coarse, scal, st));
+  DGC_LAUNCH(dgc::coarse_prolong(N, s->coarse, r, z, partial, zz, scal, st));
+  return DGC_OK;
+}
+
+extern "C" {
+
+int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell_y, int gx, int gy,
+                          int* num_coarse) {
+  DGC_REQUIRE(s && cell_x && cell_y, "dgc_ipdg_coarse_space: null argument");
+  DGC_REQUIRE(gx >= 1 && gy >= 1, "dgc_ipdg_coarse_space: need gx, gy >= 1");
+  dgc_disc_s* d = s->disc;
+  const int K = d->K, Np = d->Np, N = d->order;
+  const size_t n = size_t(K) * Np;
+  hipStream_t st = d->ctx->stream;
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  release_coarse(s);
+  // aggregates: the cells whose centroid falls in one box of a gx x gy grid
+  // over the bounding box of the vertices; empty boxes are dropped
+  double x0 = cell_x[0], x1 = cell_x[0], y0 = cell_y[0], y1 = cell_y[0];
+  for (size_t i = 0; i < 3 * size_t(K); ++i) {
+    DGC_REQUIRE(std::isfinite(cell_x[i]) && std::isfinite(cell_y[i]), "dgc_ipdg_coarse_space: bad vertex");
+    x0 = std::min(x0, cell_x[i]);
+    x1 = std::max(x1, cell_x[i]);
+    y0 = std::min(y0, cell_y[i]);
+    y1 = std::max(y1, cell_y[i]);
+  }
+  const double wx = (x1 - x0) / gx, wy = (y1 - y0) / gy;
+  DGC_REQUIRE(wx > 0 && wy > 0, "dgc_ipdg_coarse_space: degenerate mesh extent");
+  std::vector<int> box(K), box_agg(size_t(gx) * gy, -1);
+  for (int c = 0; c < K; ++c) {
+    const double cx = (cell_x[3 * c] + cell_x[3 * c + 1] + cell_x[3 * c + 2]) / 3.0;
+    const double cy = (cell_y[3 * c] + cell_y[3 * c + 1] + cell_y[3 * c + 2]) / 3.0;
+    const int ix = std::min(gx - 1, std::max(0, int((cx - x0) / wx)));
+    const int iy = std::min(gy - 1, std::max(0, int((cy - y0) / wy)));
+    box[c] = ix + gx * iy;
+    box_agg[box[c]] = 0;
+  }
+  int na = 0;
+  for (auto& b : box_agg)
+    if (b == 0) b = na++;
+  const int nc = 3 * na, ld = (nc + 3) / 4 * 4;
+  DGC_REQUIRE(nc <= 6144, "dgc_ipdg_coarse_space: more than 2048 aggregates (use a coarser grid)");
+  std::vector<int> cell_agg(K), ptr(na + 1, 0), cells(K);
+  for (int c = 0; c < K; ++c) ptr[(cell_agg[c] = box_agg[box[c]]) + 1]++;
+  for (int a = 0; a < na; ++a) ptr[a + 1] += ptr[a];
+  {
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int c = 0; c < K; ++c) cells[fill[cell_agg[c]]++] = c;
+  }
+  // basis coefficients: x(rho, sigma) = (v2 + v3)/2 + rho (v2 - v1)/2 + sigma (v3 - v1)/2
+  std::vector<float> coef(6 * size_t(K));
+  const double h = 0.5 * std::max(wx, wy);
+  for (int c = 0; c < K; ++c) {
+    const int b = box[c], ix = b % gx, iy = b / gx;
+    const double xa = x0 + (ix + 0.5) * wx, ya = y0 + (iy + 0.5) * wy;
+    const double* vx = cell_x + 3 * size_t(c);
+    const double* vy = cell_y + 3 * size_t(c);
+    const double t[6] = {(0.5 * (vx[1] + vx[2]) - xa) / h, 0.5 * (vx[1] - vx[0]) / h, 0.5 * (vx[2] - vx[0]) / h,
+                         (0.5 * (vy[1] + vy[2]) - ya) / h, 0.5 * (vy[1] - vy[0]) / h, 0.5 * (vy[2] - vy[0]) / h};
+    for (int k = 0; k < 6; ++k) coef[size_t(k) * K + c] = float(t[k]);
+  }
+  const dgc_host::RefTriangle ref(N, dgc_host::kStrongForm);
+  std::vector<double> rs(2 * size_t(Np));
+  for (int j = 0; j < Np; ++j) {
+    rs[j] = ref.r[j];
+    rs[Np + j] = ref.s[j];
+  }
+  dgc::CoarseSpace& cs = s->coarse;
+  cs.K = K;
+  cs.na = na;
+  cs.nc = nc;
+  cs.ld = ld;
+  int *dptr = nullptr, *dcells = nullptr, *dagg = nullptr;
+  float *dcoef = nullptr, *dAinv = nullptr;
+  double* drs = nullptr;
+  int rc = DGC_OK;
+  if ((rc = coarse_alloc(s, &dptr, ptr.size())) || (rc = coarse_alloc(s, &dcells, size_t(K))) ||
+      (rc = coarse_alloc(s, &dagg, size_t(K))) || (rc = coarse_alloc(s, &dcoef, coef.size())) ||
+      (rc = coarse_alloc(s, &drs, rs.size())) || (rc = coarse_alloc(s, &dAinv, size_t(nc) * ld)) ||
+      (rc = coarse_alloc(s, &cs.rc, size_t(ld))) || (rc = coarse_alloc(s, &cs.yc, size_t(ld)))) {
+    release_coarse(s);
+    return rc;
+  }
+  DGC_HIP_TRY(hipMemcpy(dptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(dagg, cell_agg.data(), cell_agg.size() * sizeof(int), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(dcoef, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(drs, rs.data(), rs.size() * sizeof(double), hipMemcpyHostToDevice));
+  cs.agg_ptr = dptr;
+  cs.agg_cells = dcells;
+  cs.cell_agg = dagg;
+  cs.coef = dcoef;
+  cs.rs = drs;
+  cs.Ainv = dAinv;
+  // Galerkin matrix column by column: A[:, col] = Phi^T S phi_col
+  double *A = nullptr, *u = nullptr, *w = nullptr, *work = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(A);
+    (void)hipFree(u);
+    (void)hipFree(w);
+    (void)hipFree(work);
+  };
+  if ((rc = alloc(&A, size_t(nc) * nc)) || (rc = alloc(&u, n)) || (rc = alloc(&w, n)) ||
+      (rc = alloc(&work, 2 * size_t(nc)))) {
+    cleanup();
+    release_coarse(s);
+    return rc;
+  }
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = u;
+  pr.out = w;
+  hipError_t e = hipSuccess;
+  for (int col = 0; col < nc && e == hipSuccess; ++col) {
+    e = hipMemsetAsync(u, 0, n * sizeof(double), st);
+    if (e == hipSuccess) e = dgc::coarse_fill(N, cs, col, u, st);
+    if (e == hipSuccess) e = dgc::ipdg_apply<double>(N, pr, dgc::kSpd, st);
+    if (e == hipSuccess) e = dgc::coarse_restrict(N, cs, w, A + col, nc, nullptr, st);
+  }
+  if (e == hipSuccess) {
+    s->coarse_A.resize(size_t(nc) * nc);
+    e = hipMemcpyAsync(s->coarse_A.data(), A, s->coarse_A.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = dgc::spd_inverse_inplace(A, nc, work, st);
+  if (e == hipSuccess) e = dgc::symmetrize_to_float(A, nc, ld, dAinv, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess) {
+    release_coarse(s);
+    return dgc_set_error(DGC_ERR_HIP, std::string("dgc_ipdg_coarse_space: ") + hipGetErrorString(e));
+  }
+  for (int i = 0; i < nc; ++i)
+    if (!(s->coarse_A[size_t(i) * nc + i] > 0)) {
+      release_coarse(s);
+      return dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_coarse_space: coarse matrix is not positive definite");
+    }
+  std::vector<float> hinv(size_t(nc) * ld);
+  DGC_HIP_TRY(hipMemcpy(hinv.data(), dAinv, hinv.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nc; ++i)
+    if (!(hinv[size_t(i) * ld + i] > 0.f) || !std::isfinite(hinv[size_t(i) * ld + i])) {
+      release_coarse(s);
+      return dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_coarse_space: coarse inverse is not positive definite");
+    }
+  if (num_coarse) *num_coarse = nc;
+  return DGC_OK;
+}
+
+int dgc_ipdg_coarse_info(dgc_ipdg_t s, double* A, float* Ainv, int* cell_agg, float* coef, double* rs) {
+  DGC_REQUIRE(s, "dgc_ipdg_coarse_info: null solver");
+  const dgc::CoarseSpace& cs = s->coarse;
+  DGC_REQUIRE(cs.nc > 0, "dgc_ipdg_coarse_info: no coarse space");
+  const int K = cs.K, nc = cs.nc;
+  if (A) std::copy(s->coarse_A.begin(), s->coarse_A.end(), A);
+  if (Ainv) DGC_HIP_TRY(hipMemcpy2D(Ainv, nc * sizeof(float), cs.Ainv, cs.ld * sizeof(float), nc * sizeof(float),
+                                    nc, hipMemcpyDeviceToHost));
+  if (cell_agg) DGC_HIP_TRY(hipMemcpy(cell_agg, cs.cell_agg, K * sizeof(int), hipMemcpyDeviceToHost));
+  if (coef) DGC_HIP_TRY(hipMemcpy(coef, cs.coef, 6 * size_t(K) * sizeof(float), hipMemcpyDeviceToHost));
+  if (rs) DGC_HIP_TRY(hipMemcpy(rs, cs.rs, 2 * size_t(s->disc->Np) * sizeof(double), hipMemcpyDeviceToHost));
+  return DGC_OK;
+}
+
 int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxit, int precond,
                  int* iters, double* relres, double* history, float* ms) {
   DGC_REQUIRE(s && b && x, "dgc_ipdg_pcg: null argument");
   DGC_REQUIRE(rtol > 0 && maxit > 0, "dgc_ipdg_pcg: need rtol > 0 and maxit > 0");
-  DGC_REQUIRE(precond == 0 || precond == 1, "dgc_ipdg_pcg: precond must be 0 (none) or 1 (block-Jacobi)");
-  if (precond == 1 && !s->f64.P) {
+  DGC_REQUIRE(precond >= 0 && precond <= 2,
+              "dgc_ipdg_pcg: precond must be 0 (none), 1 (block-Jacobi) or 2 (two-level)");
+  DGC_REQUIRE(precond != 2 || s->coarse.nc > 0, "dgc_ipdg_pcg: two-level needs dgc_ipdg_coarse_space first");
+  if (precond >= 1 && !s->f64.P) {
     int rc = dgc_ipdg_block_jacobi(s, nullptr);
     if (rc) return rc;
   }
@@ -499,7 +704,8 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
   hipStream_t st = d->ctx->stream;
   const size_t n = size_t(d->K) * d->Np;
   IpdgLevel<double>& L = s->f64;
-  L.use_P = precond == 1;
+  L.use_P = precond >= 1;
+  L.use_coarse = precond == 2;
   if (s->hist_len < maxit + 1) {
     (void)hipFree(s->hist);
     s->hist = nullptr;

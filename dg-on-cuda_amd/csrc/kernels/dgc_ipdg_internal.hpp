@@ -26,9 +26,11 @@ struct IpdgLevel {
   T* scal = nullptr;
   T* P = nullptr;  // block-Jacobi inverse blocks [Np*Np][K]
   bool use_P = false;
+  bool use_coarse = false;  // add the coarse-space correction (FP64 level only)
   hipGraphExec_t graph = nullptr;
   double* graph_hist = nullptr;
   bool graph_P = false;
+  bool graph_coarse = false;
 
   int init(const dgc_ipdg_s& s, const std::vector<double>& drT, const std::vector<double>& dsT,
            const std::vector<double>& lT, const std::vector<double>& mT, const std::vector<double>& geo,
@@ -49,7 +51,15 @@ struct dgc_ipdg_s {
   int hist_len = 0;
   IpdgLevel<double> f64;
   IpdgLevel<float> f32;
+  // coarse space of the two-level preconditioner (device arrays owned here)
+  dgc::CoarseSpace coarse;
+  std::vector<void*> coarse_owned;
+  std::vector<double> coarse_A;  // Phi^T S Phi (host copy, row-major)
+  int coarse_gen = 0;            // bumped whenever the coarse arrays change
 };
+
+// z += Phi A_c^{-1} Phi^T r with partials (r.z, r.r) or (z.z, r.r) into `partial`.
+int coarse_correct(dgc_ipdg_s* s, const double* r, double* z, double* partial, int zz, const double* scal);
 
 // FP64 block-Jacobi PCG continuing from the current iterate f64.x.
 int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, int maxit, double* history);

@@ -65,9 +65,11 @@ struct dgc_mpnn_s {
   double* wbuf = nullptr;         // S z
   double* dotpart = nullptr;      // [256] + 1 result slot
   int adam_step = 0;
+  int base = 1;  // 1: block-Jacobi, 2: two-level (block-Jacobi + coarse correction)
   double ct = 1.0;
   hipGraphExec_t graph = nullptr;
   double* graph_hist = nullptr;
+  int graph_gen = -1;
   std::vector<void*> owned;
 };
 
@@ -179,7 +181,7 @@ int host_loss(dgc_mpnn_s* m, double* loss) {
   return DGC_OK;
 }
 
-// zb = P r, partials (z.z, r.r) in L.partial
+// zb = P_BJ r, partials (z.z, r.r) in L.partial
 int bj_apply(dgc_mpnn_s* m, const double* r, double* zb) {
   IpdgLevel<double>& Lv = m->s->f64;
   dgc::CgParams<double> cp{};
@@ -197,16 +199,24 @@ int bj_apply(dgc_mpnn_s* m, const double* r, double* zb) {
   return DGC_OK;
 }
 
+// The network's base step zb = P_BJ r (+ the coarse correction for the
+// two-level base), partials (z.z, r.r) in L.partial.
+int base_apply(dgc_mpnn_s* m, const double* r, double* zb) {
+  int rc = bj_apply(m, r, zb);
+  if (rc == DGC_OK && m->base == 2) rc = coarse_correct(m->s, r, zb, m->s->f64.partial, 1, nullptr);
+  return rc;
+}
+
 int precondition(dgc_mpnn_s* m, const double* r, const double* zb, const double* q, const double* cgscal);
 
-// Energy-norm step for one pair (r = S e, er = e.r): z = P_BJ r + s_z c_t NN,
+// Energy-norm step for one pair (r = S e, er = e.r): z = zb + s_z c_t NN,
 // w = S z, loss = (z.w - 2 z.r + er) / er; with `grad`, the parameter
 // gradient goes to m->grad. *loss (host) is filled if non-null.
 int energy_step(dgc_mpnn_s* m, const double* r, double er, bool grad, double* loss) {
   hipStream_t st = stream_of(m);
   IpdgLevel<double>& Lv = m->s->f64;
   const double n = double(m->K) * m->Np;
-  int rc = bj_apply(m, r, Lv.z);
+  int rc = base_apply(m, r, Lv.z);
   if (rc) return rc;
   DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, n, nullptr, st));
   if ((rc = precondition(m, r, Lv.z, nullptr, nullptr))) return rc;
@@ -254,14 +264,17 @@ int precondition(dgc_mpnn_s* m, const double* r, const double* zb, const double*
 
 extern "C" {
 
-int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, dgc_mpnn_t* out) {
+int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, int base, dgc_mpnn_t* out) {
   DGC_REQUIRE(s && out, "dgc_mpnn_create: null argument");
+  DGC_REQUIRE(base == 1 || base == 2, "dgc_mpnn_create: base must be 1 (block-Jacobi) or 2 (two-level)");
+  DGC_REQUIRE(base == 1 || s->coarse.nc > 0, "dgc_mpnn_create: the two-level base needs dgc_ipdg_coarse_space");
   if (!s->f64.P) {
     int rc = dgc_ipdg_block_jacobi(s, nullptr);
     if (rc) return rc;
   }
   auto* m = new dgc_mpnn_s;
   m->s = s;
+  m->base = base;
   m->K = s->disc->K;
   m->N = s->disc->order;
   m->Np = s->disc->Np;
@@ -457,8 +470,8 @@ int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_
     DGC_LAUNCH(dgc::dot_device(m->e, r, n, m->dotpart, m->dotpart + 256, st));
     DGC_HIP_TRY(hipMemcpyAsync(&m->ds_er[smp], m->dotpart + 256, sizeof(double), hipMemcpyDeviceToHost, st));
     if (smp == 0) {
-      // zb = P_BJ r and the input scales for the target-scale estimate
-      int rc = bj_apply(m, r, zb);
+      // zb (the base step) and the input scales for the target-scale estimate
+      int rc = base_apply(m, r, zb);
       if (rc) return rc;
       DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, double(n), nullptr, st));
       // target scale c_t: RMS of (e - zb) / s_z on the first sample
@@ -537,18 +550,21 @@ int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss) {
 }
 
 // Flexible CG (one-term recurrence, p_{k+1} = z_{k+1} - (z_{k+1}.q_k)/(p_k.q_k) p_k)
-// with z = P_BJ r + s_z c_t NN(r, P_BJ r). Runs in graph-captured chunks; falls
-// back to block-Jacobi PCG from the current iterate if the residual stalls.
+// with z = zb + s_z c_t NN(r, zb), zb the base step (block-Jacobi or
+// two-level). Runs in graph-captured chunks; falls back to PCG with the base
+// preconditioner from the current iterate if the residual stalls.
 int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double rtol, int maxit, int stall_window,
                  double stall_factor, int* iters, double* relres, int* fell_back, double* history, float* ms) {
   DGC_REQUIRE(s && m && b && x && m->s == s, "dgc_ipdg_fcg: bad argument");
   DGC_REQUIRE(rtol > 0 && maxit > 0, "dgc_ipdg_fcg: need rtol > 0 and maxit > 0");
   DGC_REQUIRE(m->ct > 0, "dgc_ipdg_fcg: the network has no target scale (build a dataset first)");
+  DGC_REQUIRE(m->base == 1 || s->coarse.nc > 0, "dgc_ipdg_fcg: the coarse space was released");
   dgc_disc_s* d = s->disc;
   hipStream_t st = d->ctx->stream;
   const size_t n = size_t(d->K) * d->Np;
   IpdgLevel<double>& Lv = s->f64;
   Lv.use_P = true;
+  Lv.use_coarse = m->base == 2;
   if (s->hist_len < maxit + 1) {
     (void)hipFree(s->hist);
     s->hist = nullptr;
@@ -577,7 +593,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   DGC_HIP_TRY(hipMemsetAsync(Lv.x, 0, n * sizeof(double), st));
   DGC_HIP_TRY(hipMemsetAsync(Lv.p, 0, n * sizeof(double), st));
   DGC_HIP_TRY(hipMemcpyAsync(Lv.r, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  int rc = bj_apply(m, Lv.r, Lv.z);
+  int rc = base_apply(m, Lv.r, Lv.z);
   if (rc) return rc;
   DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, Lv.scal, m->mscal, nd, s->hist, st));
   if ((rc = precondition(m, Lv.r, Lv.z, nullptr, Lv.scal))) return rc;
@@ -606,6 +622,10 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     cp.partial = Lv.partial;
     cp.zz = 1;
     DGC_LAUNCH(dgc::cg_update_launch<double>(d->order, cp, st));
+    if (m->base == 2) {
+      int r1 = coarse_correct(s, Lv.r, Lv.z, Lv.partial, 1, Lv.scal);
+      if (r1) return r1;
+    }
     DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 0, Lv.scal, m->mscal, nd, s->hist, st));
     int r2 = precondition(m, Lv.r, Lv.z, Lv.q, Lv.scal);
     if (r2) return r2;
@@ -614,7 +634,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     return DGC_OK;
   };
   const int chunk = 8;
-  if (!m->graph || m->graph_hist != s->hist) {
+  if (!m->graph || m->graph_hist != s->hist || m->graph_gen != s->coarse_gen) {
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
     m->graph = nullptr;
     hipGraph_t g = nullptr;
@@ -633,7 +653,12 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
       return dgc_set_error(DGC_ERR_HIP, std::string("fcg instantiate: ") + hipGetErrorString(ce));
     }
     m->graph_hist = s->hist;
+    m->graph_gen = s->coarse_gen;
   }
+  // Stall test on the running minimum of the squared residual (CG residuals
+  // are not monotone): fall back when the best value of the last
+  // stall_window iterations is not below stall_factor^2 times the best value
+  // before them.
   std::vector<double> hist_h;
   int it = 0, fb = 0;
   for (;;) {
@@ -641,11 +666,19 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     DGC_HIP_TRY(hipStreamSynchronize(st));
     it = int(h[dgc::kIter]);
     if (h[dgc::kDone] != 0.0 || it >= maxit) break;
-    if (stall_window > 0 && it >= stall_window) {
+    if (!(h[dgc::kRr] == h[dgc::kRr])) {
+      fb = 1;
+      break;
+    }
+    if (stall_window > 0 && it >= 2 * stall_window) {
+      const size_t have = hist_h.size();
       hist_h.resize(it + 1);
-      DGC_HIP_TRY(hipMemcpy(hist_h.data(), s->hist, (it + 1) * sizeof(double), hipMemcpyDeviceToHost));
-      const double now = hist_h[it], before = hist_h[it - stall_window];
-      if (!(now == now) || now > stall_factor * stall_factor * before) {
+      DGC_HIP_TRY(hipMemcpy(hist_h.data() + have, s->hist + have, (it + 1 - have) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+      const auto split = hist_h.begin() + (it + 1 - stall_window);
+      const double best_before = *std::min_element(hist_h.begin(), split);
+      const double best_recent = *std::min_element(split, hist_h.end());
+      if (best_recent > stall_factor * stall_factor * best_before) {
         fb = 1;
         break;
       }
@@ -654,17 +687,18 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   }
   const bool nan_state = !(h[dgc::kRr] == h[dgc::kRr]);
   if (nan_state) {
-    // the network produced non-finite values: restart from zero with block-Jacobi PCG
+    // the network produced non-finite values: restart from zero with the base preconditioner
     DGC_HIP_TRY(hipMemsetAsync(Lv.x, 0, n * sizeof(double), st));
-    fb = 1;
   }
-  // true FP64 residual; fall back to block-Jacobi PCG from x if needed
+  // true FP64 residual; continue with PCG on the base preconditioner from x
+  // after a stall, or restart it while recurrence drift leaves the true
+  // residual above rtol
   auto true_rel = [&](double& rel) -> int {
     dgc::IpdgParams<double> pr = Lv.params(*s);
     pr.u = Lv.x;
     pr.out = Lv.q;
     DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
-    std::vector<double> hq(n), hx(n);
+    std::vector<double> hq(n);
     DGC_HIP_TRY(hipMemcpyAsync(hq.data(), Lv.q, n * sizeof(double), hipMemcpyDeviceToHost, st));
     DGC_HIP_TRY(hipStreamSynchronize(st));
     double rr = 0.0;
@@ -674,7 +708,8 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   };
   double rel = 1.0;
   if ((rc = true_rel(rel))) return rc;
-  if ((fb || rel > rtol) && it < maxit) {
+  bool more = fb || !(rel <= rtol);
+  for (int restart = 0; more && it < maxit && restart < 4; ++restart) {
     if (!fb) fb = 2;  // finished by a PCG restart after recurrence drift
     rc = cg_init_fp64_restart(s, b, rtol * rtol * bb, maxit - it, s->hist + it);
     int it2 = 0;
@@ -682,6 +717,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     if (rc) return rc;
     it += it2;
     if ((rc = true_rel(rel))) return rc;
+    more = !(rel <= rtol);
   }
   DGC_HIP_TRY(hipEventRecord(e1, st));
   DGC_HIP_TRY(hipEventSynchronize(e1));

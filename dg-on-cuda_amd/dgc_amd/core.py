@@ -78,6 +78,8 @@ def _declare(L):
     d("dgc_ipdg_mass", [_vp, _vp, _vp])
     d("dgc_ipdg_block_jacobi", [_vp, _ip])
     d("dgc_ipdg_diag_blocks", [_vp, _dp])
+    d("dgc_ipdg_coarse_space", [_vp, _dp, _dp, C.c_int, C.c_int, _ip])
+    d("dgc_ipdg_coarse_info", [_vp, _dp, C.POINTER(C.c_float), _ip, C.POINTER(C.c_float), _dp])
     d("dgc_ipdg_pcg", [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _ip, _dp, _dp,
                        C.POINTER(C.c_float)])
     d("dgc_ipdg_pcg_mixed", [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, _ip, _ip, _dp,

@@ -54,8 +54,32 @@ class IPDG:
         check(lib().dgc_ipdg_diag_blocks(self._h, _d(b)))
         return b.reshape(Np, Np, K).transpose(2, 0, 1).copy()
 
+    def coarse_space(self, mesh, gx: int, gy: int | None = None) -> int:
+        """Two-level coarse space: linear functions on gx x gy boxes of cells."""
+        c = mesh.cells()
+        n = C.c_int()
+        check(lib().dgc_ipdg_coarse_space(self._h, _d(np.ascontiguousarray(c["cx"], np.float64)),
+                                          _d(np.ascontiguousarray(c["cy"], np.float64)), gx,
+                                          gy if gy is not None else gx, C.byref(n)))
+        self.num_coarse = n.value
+        return n.value
+
+    def coarse_info(self) -> dict:
+        nc, K, Np = self.num_coarse, self.disc.K, self.disc.Np
+        A = np.zeros(nc * nc)
+        Ainv = np.zeros(nc * nc, np.float32)
+        agg = np.zeros(K, np.int32)
+        coef = np.zeros(6 * K, np.float32)
+        rs = np.zeros(2 * Np)
+        check(lib().dgc_ipdg_coarse_info(self._h, _d(A), Ainv.ctypes.data_as(C.POINTER(C.c_float)),
+                                         agg.ctypes.data_as(C.POINTER(C.c_int)),
+                                         coef.ctypes.data_as(C.POINTER(C.c_float)), _d(rs)))
+        return dict(A=A.reshape(nc, nc), Ainv=Ainv.reshape(nc, nc), cell_agg=agg, coef=coef.reshape(6, K),
+                    r=rs[:Np], s=rs[Np:])
+
     def pcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-10, maxit=20000, precond=1,
             history=False) -> dict:
+        """precond 0: none, 1: block-Jacobi, 2: two-level (after coarse_space)."""
         it = C.c_int()
         rel = C.c_double()
         ms = C.c_float()

@@ -9,6 +9,7 @@ from .core import DeviceArray, _vp, check, lib
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
+_NOT_CONVERGED = 5  # DGC_ERR_NOT_CONVERGED
 
 
 def _declare():
@@ -21,7 +22,7 @@ def _declare():
         f.argtypes = args
         f.restype = res
 
-    d("dgc_mpnn_create", [_vp, C.c_uint64, C.POINTER(_vp)])
+    d("dgc_mpnn_create", [_vp, C.c_uint64, C.c_int, C.POINTER(_vp)])
     d("dgc_mpnn_destroy", [_vp])
     d("dgc_mpnn_param_count", [_vp])
     d("dgc_mpnn_input_width", [_vp])
@@ -42,14 +43,16 @@ def _declare():
 
 
 class LearnedPreconditioner:
-    """3-layer message-passing network correcting the block-Jacobi step."""
+    """3-layer message-passing network correcting a base preconditioner step:
+    base 1 = block-Jacobi, base 2 = two-level (needs ipdg.coarse_space first)."""
 
-    def __init__(self, ipdg, seed: int = 0):
+    def __init__(self, ipdg, seed: int = 0, base: int = 1):
         L = _declare()
         self.ipdg = ipdg
         self.ctx = ipdg.ctx
+        self.base = base
         h = _vp()
-        check(L.dgc_mpnn_create(ipdg._h, seed, C.byref(h)))
+        check(L.dgc_mpnn_create(ipdg._h, seed, base, C.byref(h)))
         self._h = h
         self.P = L.dgc_mpnn_param_count(h)
         self.D0 = L.dgc_mpnn_input_width(h)
@@ -116,17 +119,21 @@ class LearnedPreconditioner:
         check(lib().dgc_mpnn_eval_loss(self._h, first, count, C.byref(v)))
         return v.value
 
-    def fcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-10, maxit=20000, stall_window=200,
-            stall_factor=0.99, history=False) -> dict:
+    def fcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-10, maxit=20000, stall_window=1000,
+            stall_factor=0.99, history=False, raise_on_cap=True) -> dict:
+        """Flexible CG; with raise_on_cap=False an iteration cap is reported as
+        converged=False instead of raising."""
         it = C.c_int()
         rel = C.c_double()
         fb = C.c_int()
         ms = C.c_float()
         hist = np.zeros(maxit + 1) if history else None
-        check(lib().dgc_ipdg_fcg(self.ipdg._h, self._h, _vp(b.ptr), _vp(x.ptr), rtol, maxit, stall_window,
-                                 stall_factor, C.byref(it), C.byref(rel), C.byref(fb),
-                                 hist.ctypes.data_as(_dp) if history else None, C.byref(ms)))
-        out = dict(iters=it.value, relres=rel.value, fell_back=fb.value, ms=ms.value)
+        rc = lib().dgc_ipdg_fcg(self.ipdg._h, self._h, _vp(b.ptr), _vp(x.ptr), rtol, maxit, stall_window,
+                                stall_factor, C.byref(it), C.byref(rel), C.byref(fb),
+                                hist.ctypes.data_as(_dp) if history else None, C.byref(ms))
+        if raise_on_cap or rc != _NOT_CONVERGED:
+            check(rc)
+        out = dict(iters=it.value, relres=rel.value, fell_back=fb.value, ms=ms.value, converged=rc == 0)
         if history:
             out["history"] = np.sqrt(hist[: it.value + 1])
         return out

@@ -184,10 +184,26 @@ int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out);
 int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);
 /* Copies the diagonal blocks [Np*Np][K] (entry (j,k) of cell c at (j*Np+k)*K+c). */
 int dgc_ipdg_diag_blocks(dgc_ipdg_t s, double* blocks);
+/* Coarse space of the two-level preconditioner (no reference counterpart):
+ * cells are grouped by the box of a gx x gy grid over the mesh extent that
+ * holds their centroid; each non-empty group carries the linear functions
+ * 1, (x - x_a)/h, (y - y_a)/h (cell_x, cell_y: host vertex coordinates
+ * [3K], the order the discretization's geometry was built from). The
+ * Galerkin matrix A_c = Phi^T S Phi is formed with the operator itself and
+ * inverted on the device; the inverse is kept in FP32. At most 2048 groups.
+ * *num_coarse = 3 x groups. */
+int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell_y, int gx, int gy,
+                          int* num_coarse);
+/* Host copies (each may be NULL): A_c [nc*nc] (FP64), its FP32 inverse
+ * [nc*nc], the group of each cell [K], the basis coefficients [6][K] and the
+ * reference node coordinates (r, s) [2][Np]. */
+int dgc_ipdg_coarse_info(dgc_ipdg_t s, double* A, float* Ainv, int* cell_agg, float* coef, double* rs);
 /* Conjugate gradients on (-M Lap_h) x = b to ||b - S x|| <= rtol ||b|| (FP64
  * true residual, restarted if the recurrence drifts). precond 0: none,
- * 1: block-Jacobi. x: device output (initial guess 0). history (host, may be
- * NULL): maxit+1 squared recurrence residuals. *ms: solve time (HIP events).
+ * 1: block-Jacobi, 2: two-level (block-Jacobi + coarse correction,
+ * z = P_BJ r + Phi A_c^{-1} Phi^T r). x: device output (initial guess 0).
+ * history (host, may be NULL): maxit+1 squared recurrence residuals.
+ * *ms: solve time (HIP events).
  * Replaces the assembly + sparse direct solve of examples/poisson_2d/main.cpp:132-223. */
 int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxit, int precond,
                  int* iters, double* relres, double* history, float* ms);
@@ -207,7 +223,9 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
  * seeded random error fields e, with the energy-norm loss
  * |z - e|_S^2 / |e|_S^2 (the error CG minimises), and used inside flexible CG. */
 typedef struct dgc_mpnn_s* dgc_mpnn_t;
-int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, dgc_mpnn_t* out);
+/* base 1: zb = P_BJ r; base 2: zb = the two-level step (needs
+ * dgc_ipdg_coarse_space first; the coarse space must outlive the network). */
+int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, int base, dgc_mpnn_t* out);
 int dgc_mpnn_destroy(dgc_mpnn_t m);
 int dgc_mpnn_param_count(dgc_mpnn_t m);
 /* width D0 of the input feature vector per cell (2 Np + 4 rounded up to 8) */
@@ -240,8 +258,9 @@ int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss);
 /* Flexible CG on (-M Lap_h) x = b with the network preconditioner, to the FP64
  * true relative residual rtol. If the squared residual fails to drop below
  * stall_factor^2 times its value stall_window iterations earlier (or turns
- * non-finite), the solve continues with block-Jacobi PCG from the current
- * iterate (*fell_back = 1; 2 = finished by a PCG restart after drift). */
+ * non-finite), the solve continues with PCG on the network's base
+ * preconditioner (block-Jacobi or two-level) from the current iterate
+ * (*fell_back = 1; 2 = finished by a PCG restart after drift). */
 int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double rtol, int maxit,
                  int stall_window, double stall_factor, int* iters, double* relres, int* fell_back,
                  double* history, float* ms);

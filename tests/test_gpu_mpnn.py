@@ -138,25 +138,29 @@ def _oracle_relres(mesh, disc, x_node, b_node):
     return np.linalg.norm(b - Sx) / np.linalg.norm(b)
 
 
-def test_train_and_flexible_cg(ctx):
+@pytest.mark.parametrize("base", [1, 2])
+def test_train_and_flexible_cg(ctx, base):
+    """Untrained (zero output layer) FCG reproduces PCG with the base
+    preconditioner; training lowers the held-out energy loss; FCG with the
+    trained network reaches the FP64 tolerance by the oracle's residual."""
     mesh, disc, op, b = _poisson(ctx, 32, 3)
-    net = dgc_amd.LearnedPreconditioner(op, seed=0)
-    # untrained: the output layer is zero, so FCG is block-Jacobi CG
+    if base == 2:
+        op.coarse_space(mesh, 8)
+    net = dgc_amd.LearnedPreconditioner(op, seed=0, base=base)
     net.build_dataset(mesh, 96, seed0=100, modes=16)
     x0 = dgc_amd.DeviceArray(ctx, disc.size)
     r0 = net.fcg(b, x0, rtol=1e-10)
+    xp = dgc_amd.DeviceArray(ctx, disc.size)
+    rp = op.pcg(b, xp, rtol=1e-10, precond=base)
     before = net.eval_loss(64, 32)
-    hist = net.train(0, 64, epochs=4, lr=2e-3, seed=0)
+    hist = net.train(0, 64, epochs=4, lr=3e-4, seed=0)
     after = net.eval_loss(64, 32)
     x1 = dgc_amd.DeviceArray(ctx, disc.size)
     r1 = net.fcg(b, x1, rtol=1e-10)
-    xp = dgc_amd.DeviceArray(ctx, disc.size)
-    rp = op.pcg(b, xp, rtol=1e-10)
     rr = _oracle_relres(mesh, disc, x1.download(), b.download())
-    print(f"train loss {hist}, held-out {before:.4f} -> {after:.4f}; FCG untrained {r0['iters']} it "
-          f"{r0['ms']:.1f} ms, trained {r1['iters']} it {r1['ms']:.1f} ms (fallback {r1['fell_back']}); "
-          f"PCG {rp['iters']} it {rp['ms']:.1f} ms; oracle relres {rr:.2e}")
+    print(f"base {base}: train loss {np.round(hist, 4)}, held-out {before:.4f} -> {after:.4f}; FCG untrained "
+          f"{r0['iters']} it {r0['ms']:.1f} ms, trained {r1['iters']} it {r1['ms']:.1f} ms (fallback "
+          f"{r1['fell_back']}); PCG {rp['iters']} it {rp['ms']:.1f} ms; oracle relres {rr:.2e}")
+    assert abs(r0["iters"] - rp["iters"]) <= 2
     assert after < before
-    assert hist[-1] < hist[0]
     assert r1["relres"] <= 1e-10 and rr <= 2e-10
-    assert abs(r0["iters"] - rp["iters"]) <= 0.05 * rp["iters"] + 5
