@@ -10,10 +10,13 @@ sweep's whole-job rate is  sum_N (K Np 4 steps) / sum_N (wall time).
 
 Alongside: the roofline of the dominant kernel (HIP events around each stage
 kernel; algorithmic bytes per launch), an HBM-resident variant (K = 2.1M at
-N = 3, larger than the 256 MB Infinity Cache), and the reference CPU path
-timed on a bounded sample on the host cores.
+N = 3, larger than the 256 MB Infinity Cache), the reference CPU path
+timed on a bounded sample on the host cores, and the IPDG Poisson solvers'
+time to FP64 relative residual 1e-10 (block-Jacobi PCG, two-level PCG and
+flexible CG with the learned preconditioner) at K = 32768 N = 3 and
+K = 131072 N = 2 over held-out right-hand sides.
 
-    python bench.py [--steps 200] [--warmup 10] [--gpus 1]
+    python bench.py [--steps 200] [--warmup 10] [--gpus 1] [--ipdg-full]
 """
 from __future__ import annotations
 
@@ -50,9 +53,10 @@ def setup_advection(dgc, ctx, mesh, N, a=(1.0, 0.5)):
     return disc, op, state, work, dt, u0
 
 
-def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5):
+def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64"):
     disc, op, state, work, dt, u0 = setup_advection(dgc, ctx, mesh, N)
     K, Np = disc.K, disc.Np
+    op.precision = precision
     tune = op.autotune(state, work)  # picks the tiling (waves per 64-cell tile)
     if warmup:
         op.rk4_steps(state, 0.0, dt, warmup, work)
@@ -66,7 +70,24 @@ def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5):
     ok = bool(np.all(np.isfinite(u)))
     dofs = K * Np * 4 * steps
     return dict(N=N, Np=Np, K=K, dt=dt, wall_s=wall, event_ms=ev_ms, dof_stage=dofs,
-                value=dofs / wall, stage_ms=stage_ms, finite=ok, tune=tune)
+                value=dofs / wall, stage_ms=stage_ms, finite=ok, tune=tune, u=u)
+
+
+MFMA_PEAK_F32_TFLOPS = 157.3  # MI355X dense FP32 matrix, published
+
+
+def mfma_stats(r):
+    """Matrix-core rate of the FP32 form's stage-2 kernel: issued flops (the
+    16x16x4 blocks, padding included) and useful flops (2 Np^2 + Np 3Nfp
+    multiply-adds per cell for Dr, Ds and LIFT) per launch over its mean duration."""
+    N, Np, K = r["N"], r["Np"], r["K"]
+    NF = 3 * (N + 1)
+    MB, KS, KL = (Np + 15) // 16, (Np + 3) // 4, (NF + 3) // 4
+    issued = 2.0 * 1024 * MB * (2 * KS + KL) * ((K + 15) // 16)
+    useful = 2.0 * (2 * Np * Np + Np * NF) * K
+    dur = r["tune"]["ms"][r["tune"]["choice"]] * 1e-3
+    return dict(issued_tflops=round(issued / dur / 1e12, 3), useful_tflops=round(useful / dur / 1e12, 3),
+                peak_tflops=MFMA_PEAK_F32_TFLOPS, frac_issued=round(issued / dur / 1e12 / MFMA_PEAK_F32_TFLOPS, 4))
 
 
 def roofline_for(r, F=1):
@@ -123,6 +144,83 @@ def cpu_baseline(dgc, N=3, steps=40):
                        f"in {secs:.2f} s on 1 host core")
 
 
+def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
+    """IPDG Poisson -Lap u = f on [-1,1]^2, u = 0 on the boundary, tau = (N+1)(N+2)/h,
+    n_side x n_side squares split into triangles, jitter 0.1h (seed 3). Time to
+    FP64 relative residual 1e-10 for block-Jacobi PCG, two-level PCG (gx x gx
+    aggregates) and flexible CG with the learned preconditioner on the two-level
+    base, over held-out right-hand sides b = M f (f: 2 pi^2 sin sin + 16 Fourier
+    modes with k^-2 amplitudes, seeds 10000..). The network is trained first on
+    n_train pairs r = S e from Gaussian random fields (seeds 100..)."""
+    mesh = dgc.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=3)
+    disc = dgc.Discretization(ctx, mesh, N)
+    op = dgc.IPDG(disc, tau_mode=1)
+    t0 = time.perf_counter()
+    op.block_jacobi()
+    t_bj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nc = op.coarse_space(mesh, gx)
+    t_coarse = time.perf_counter() - t0
+    net = dgc.LearnedPreconditioner(op, seed=0, base=2)
+    t0 = time.perf_counter()
+    net.build_dataset(mesh, n_train, seed0=100, modes=16)
+    t_data = time.perf_counter() - t0
+    held = max(1, n_train // 16)
+    loss_before = net.eval_loss(n_train - held, held)
+    t0 = time.perf_counter()
+    hist = net.train(0, n_train - held, epochs=epochs, lr=3e-4, seed=0)
+    t_train = time.perf_counter() - t0
+    loss_after = net.eval_loss(n_train - held, held)
+    print(f"# IPDG K={disc.K} N={N}: coarse {gx}x{gx} ({nc} unknowns) {t_coarse:.2f} s, dataset {t_data:.1f} s, "
+          f"training {t_train:.1f} s, held-out energy loss {loss_before:.4f} -> {loss_after:.4f}",
+          file=sys.stderr, flush=True)
+    xg, yg = mesh.dof_coords(N, node_major=True)
+    b = dgc.DeviceArray(ctx, disc.size)
+    x = dgc.DeviceArray(ctx, disc.size)
+    res = {k: dict(iters=[], ms=[], relres=[]) for k in ("bj_pcg", "two_level_pcg", "learned_fcg")}
+    fallbacks = 0
+    for i in range(n_eval):
+        f, _ = dgc.poisson_rhs(xg, yg, seed=10000 + i)
+        op.mass(dgc.DeviceArray.from_host(ctx, f), b)
+        runs = (("bj_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=1)),
+                ("two_level_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=2)),
+                ("learned_fcg", lambda: net.fcg(b, x, rtol=1e-10)))
+        for name, fn in runs:
+            if i == 0:
+                fn()  # graph capture and first-touch outside the timing
+            r = fn()
+            res[name]["iters"].append(r["iters"])
+            res[name]["ms"].append(r["ms"])
+            res[name]["relres"].append(r["relres"])
+            if name == "learned_fcg" and r["fell_back"] == 1:
+                fallbacks += 1
+    out = dict(K=disc.K, N=N, n=disc.size, coarse_grid=[gx, gx], coarse_unknowns=nc,
+               setup_s=dict(block_jacobi=round(t_bj, 3), coarse=round(t_coarse, 3), dataset=round(t_data, 2),
+                            training=round(t_train, 2)),
+               training=dict(pairs=n_train - held, held_out=held, epochs=epochs,
+                             epoch_loss=[round(v, 5) for v in hist], held_out_loss_untrained=round(loss_before, 5),
+                             held_out_loss_trained=round(loss_after, 5)),
+               eval_rhs=dict(count=n_eval, seeds=[10000, 10000 + n_eval - 1]), fcg_stall_fallbacks=fallbacks)
+    for name, v in res.items():
+        out[name] = dict(median_iters=int(np.median(v["iters"])), median_ms=round(float(np.median(v["ms"])), 3),
+                         mean_ms=round(float(np.mean(v["ms"])), 3), max_relres=float(np.max(v["relres"])))
+    out["speedup_learned_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["learned_fcg"]["median_ms"], 2)
+    out["speedup_two_level_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 2)
+    out["learned_vs_two_level_ms"] = round(out["learned_fcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 3)
+    if mixed:
+        f, _ = dgc.poisson_rhs(xg, yg, seed=3)
+        op.mass(dgc.DeviceArray.from_host(ctx, f), b)
+        op.pcg_mixed(b, x, rtol=1e-10)
+        r = op.pcg_mixed(b, x, rtol=1e-10)
+        out["mixed_fp32_inner_bj"] = dict(outer=r["outer"], inner_iters=r["inner_iters"], ms=round(r["ms"], 3),
+                                         relres=r["relres"])
+    for name in ("bj_pcg", "two_level_pcg", "learned_fcg"):
+        print(f"#   {name}: median {out[name]['median_iters']} it, {out[name]['median_ms']:.2f} ms, "
+              f"max relres {out[name]['max_relres']:.2e}", file=sys.stderr, flush=True)
+    del net, op, disc
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +234,10 @@ def main():
     ap.add_argument("--calib", action="store_true",
                     help="also run dgc_axpy over 2^27 doubles: a kernel of known traffic "
                          "(reads 2 GiB, writes 1 GiB) to scale HBM byte counters")
+    ap.add_argument("--no-ipdg", action="store_true", help="skip the IPDG solver comparison")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 matrix-core form of the sweep")
+    ap.add_argument("--ipdg-full", action="store_true",
+                    help="IPDG comparison at full size: 4096 training pairs, 256 held-out right-hand sides")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -165,12 +267,26 @@ def main():
     mesh = dgc.Mesh.structured(256, 256, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
     orders = [int(o) for o in args.orders.split(",")]
     per = []
+    fp32 = {}
     for N in orders:
         r = run_order(dgc, ctx, mesh, N, args.steps, args.warmup)
         per.append(r)
         print(f"# N={N} Np={r['Np']}: {r['value'] / 1e9:.2f} G DOF-stage/s, "
               f"{r['wall_s'] * 1e3 / args.steps:.3f} ms/step, stage us "
               f"{[round(x * 1e3, 1) for x in r['stage_ms']]}", file=sys.stderr, flush=True)
+        if not args.no_fp32:
+            # FP32 matrix-core form: same steps from the same initial state
+            q = run_order(dgc, ctx, mesh, N, args.steps, args.warmup, precision="fp32-mfma")
+            err = float(np.linalg.norm(q["u"] - r["u"]) / np.linalg.norm(r["u"]))
+            fp32[N] = dict(value=q["value"], ms_per_step=q["wall_s"] * 1e3 / args.steps,
+                           stage_us=[round(x * 1e3, 2) for x in q["stage_ms"]],
+                           stage2_GBs=roofline_for(q)["achieved"], rel_l2_vs_fp64=err,
+                           speedup_vs_fp64=round(q["value"] / r["value"], 3), mfma=mfma_stats(q))
+            print(f"#   fp32-mfma: {q['value'] / 1e9:.2f} G DOF-stage/s ({fp32[N]['speedup_vs_fp64']}x), "
+                  f"rel L2 vs fp64 after {args.warmup + args.steps + 5} steps {err:.2e}, "
+                  f"MFMA {fp32[N]['mfma']['issued_tflops']} TFLOP/s issued", file=sys.stderr, flush=True)
+            del q
+        r.pop("u")
     total_dofs = sum(r["dof_stage"] for r in per)
     total_s = sum(r["wall_s"] for r in per)
     value = total_dofs / total_s
@@ -186,6 +302,13 @@ def main():
                                           "256 MB Infinity Cache with the 5 RK buffers")
         print(f"# HBM-resident K={r['K']} N=3: {r['value'] / 1e9:.2f} G DOF-stage/s, "
               f"stage-2 {roof and roofline_for(r)['achieved']} GB/s", file=sys.stderr, flush=True)
+
+    if not args.no_ipdg:
+        full = args.ipdg_full
+        extra["ipdg"] = [
+            ipdg_bench(dgc, ctx, 128, 3, 32, 4096 if full else 1024, 1, 256 if full else 24, mixed=True),
+            ipdg_bench(dgc, ctx, 256, 2, 32, 4096 if full else 512, 1, 256 if full else 6),
+        ]
 
     cpu = None
     if not args.no_cpu:
@@ -214,7 +337,8 @@ def main():
                            stage_us=[round(x * 1e3, 2) for x in r["stage_ms"]],
                            stage2_GBs=roofline_for(r)["achieved"], finite=r["finite"],
                            waves_per_tile=r["tune"]["choice"],
-                           autotune_stage2_us={k: round(v * 1e3, 2) for k, v in r["tune"]["ms"].items()})
+                           autotune_stage2_us={k: round(v * 1e3, 2) for k, v in r["tune"]["ms"].items()},
+                           fp32_mfma=fp32.get(r["N"]))
                       for r in per],
         **extra,
     }

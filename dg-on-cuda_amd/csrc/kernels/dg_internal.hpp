@@ -59,7 +59,19 @@ struct StageParams {
   double* u[3];
   double* acc[3];
   double* k3[3];
+  // FP32 matrix-core operand blocks of Dr, Ds and LIFT (advection only): when
+  // set, the stage runs the FP32-MFMA contraction kernel (dg_rk_mfma.hip).
+  const float* __restrict__ opF;
 };
+
+// Number of FP32 operand blocks (16 rows x 4 columns, 64 floats each) of the
+// MFMA stage kernel: Dr and Ds as ceil(Np/16) x ceil(Np/4) blocks, LIFT as
+// ceil(Np/16) x ceil(3Nfp/4) blocks.
+inline int mfma_operand_blocks(int Np, int Nfp) {
+  const int MB = (Np + 15) / 16, KS = (Np + 3) / 4, KL = (3 * Nfp + 3) / 4;
+  return MB * (2 * KS + KL);
+}
+hipError_t launch_stage_mfma(int order, int stage, const StageParams& p, hipStream_t s);
 
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }

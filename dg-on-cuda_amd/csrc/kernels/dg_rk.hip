@@ -354,6 +354,7 @@ int default_waves_per_tile(int order, int physics) {
 }
 
 hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s) {
+  if (p.opF != nullptr) return physics == kAdvection ? launch_stage_mfma(order, stage, p, s) : hipErrorInvalidValue;
   if (stage < 0 || stage > 4 || (wt != 1 && wt != 2 && wt != 4)) return hipErrorInvalidValue;
   wt = supported_waves_per_tile(order, physics, wt);
   StageFn fn = physics == kMaxwellTM ? pick<kMaxwellTM>(order, wt, stage) : pick<kAdvection>(order, wt, stage);

@@ -42,6 +42,9 @@ struct dgc_op_s {
   double ax = 0.0, ay = 0.0;
   int bc = 0;
   int wt = 4;  // waves per 64-cell tile of the stage kernels
+  int precision = 0;      // 0: FP64; 1: FP32 matrix-core contractions (advection)
+  float* opF = nullptr;   // MFMA operand blocks of Dr, Ds, LIFT (built on first use)
+  std::vector<double> hDr, hDs, hL;  // host copies of the operators
   double* tclock = nullptr;
   // cached graph of one full RK4 step
   hipGraphExec_t graph = nullptr;
@@ -263,11 +266,42 @@ static int upload_operators(dgc_op_s* op, const double* Dr, const double* Ds, co
   DGC_HIP_TRY(hipMemcpy(op->DrT, drT.data(), drT.size() * sizeof(double), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(op->DsT, dsT.data(), dsT.size() * sizeof(double), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(op->LT, lT.data(), lT.size() * sizeof(double), hipMemcpyHostToDevice));
+  op->hDr.assign(Dr, Dr + size_t(Np) * Np);
+  op->hDs.assign(Ds, Ds + size_t(Np) * Np);
+  op->hL.assign(lift, lift + size_t(Np) * NF);
+  return DGC_OK;
+}
+
+// FP32 operand blocks of the MFMA stage kernel: block a holds rows
+// 16 mb .. 16 mb + 15 and columns 4 ks .. 4 ks + 3 of one matrix, lane l
+// storing entry (16 mb + l % 16, 4 ks + l / 16) (zero outside the matrix);
+// blocks are ordered Dr (mb, ks), Ds (mb, ks), LIFT (mb, ks).
+static int build_mfma_operands(dgc_op_s* op) {
+  const int Np = op->disc->Np, NF = 3 * op->disc->Nfp;
+  const int MB = (Np + 15) / 16, KS = (Np + 3) / 4, KL = (NF + 3) / 4;
+  const int NA = dgc::mfma_operand_blocks(Np, op->disc->Nfp);
+  std::vector<float> h(size_t(NA) * 64, 0.f);
+  auto put = [&](int a, const std::vector<double>& M, int cols, int mb, int ks) {
+    for (int l = 0; l < 64; ++l) {
+      const int i = 16 * mb + l % 16, k = 4 * ks + l / 16;
+      if (i < Np && k < cols) h[size_t(a) * 64 + l] = float(M[size_t(i) * cols + k]);
+    }
+  };
+  for (int mb = 0; mb < MB; ++mb)
+    for (int ks = 0; ks < KS; ++ks) {
+      put(mb * KS + ks, op->hDr, Np, mb, ks);
+      put(MB * KS + mb * KS + ks, op->hDs, Np, mb, ks);
+    }
+  for (int mb = 0; mb < MB; ++mb)
+    for (int ks = 0; ks < KL; ++ks) put(2 * MB * KS + mb * KL + ks, op->hL, NF, mb, ks);
+  DGC_HIP_TRY(hipMalloc(&op->opF, h.size() * sizeof(float)));
+  DGC_HIP_TRY(hipMemcpy(op->opF, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   return DGC_OK;
 }
 
 static void free_op(dgc_op_s* op) {
   if (op->graph) hipGraphExecDestroy(op->graph);
+  hipFree(op->opF);
   hipFree(op->DrT);
   hipFree(op->DsT);
   hipFree(op->LT);
@@ -336,6 +370,7 @@ static dgc::StageParams base_params(const dgc_op_s* op) {
   p.ax = op->ax;
   p.ay = op->ay;
   p.tclock = nullptr;
+  p.opF = op->precision == 1 ? op->opF : nullptr;
   return p;
 }
 
@@ -543,6 +578,25 @@ int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
 
 int dgc_op_waves_per_tile(dgc_op_t op) { return op ? op->wt : 0; }
 
+int dgc_op_set_precision(dgc_op_t op, int precision) {
+  DGC_REQUIRE(op != nullptr, "dgc_op_set_precision: null operator");
+  DGC_REQUIRE(precision == 0 || precision == 1, "dgc_op_set_precision: precision must be 0 (FP64) or 1 (FP32 MFMA)");
+  DGC_REQUIRE(precision == 0 || op->physics == dgc::kAdvection,
+              "dgc_op_set_precision: the FP32 matrix-core form is built for the advection operator");
+  if (precision == 1 && op->opF == nullptr) {
+    int rc = build_mfma_operands(op);
+    if (rc != DGC_OK) return rc;
+  }
+  op->precision = precision;
+  if (op->graph) {
+    (void)hipGraphExecDestroy(op->graph);
+    op->graph = nullptr;
+  }
+  return DGC_OK;
+}
+
+int dgc_op_precision(dgc_op_t op) { return op ? op->precision : -1; }
+
 // Times the stage-2 kernel (the widest: 5 state passes) of every supported
 // tiling on the caller's buffers and keeps the fastest. Only work vectors are
 // written (stage 2 writes wk0 and wk2, which the next step's stage 1
@@ -573,6 +627,7 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
     const int wt = cand[i];
     if (ms_out) ms_out[i] = -1.f;
     if (dgc::supported_waves_per_tile(op->disc->order, op->physics, wt) != wt) continue;
+    if (op->precision == 1 && wt != 4) continue;  // the matrix-core kernel has one tiling
     // warm-up launch, then timed repetitions
     hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 2, wt, p, st);
     if (e == hipSuccess) e = hipEventRecord(e0, st);

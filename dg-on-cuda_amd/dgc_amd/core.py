@@ -86,6 +86,8 @@ def _declare(L):
                              C.POINTER(C.c_float)])
     d("dgc_op_set_waves_per_tile", [_vp, C.c_int])
     d("dgc_op_waves_per_tile", [_vp])
+    d("dgc_op_set_precision", [_vp, C.c_int])
+    d("dgc_op_precision", [_vp])
     d("dgc_op_autotune", [_vp, C.POINTER(_vp), C.c_size_t, C.POINTER(_vp), C.c_int, C.POINTER(C.c_float)])
     for name, args in _EXTRA_DECLS:
         d(name, args)
@@ -398,6 +400,15 @@ class _Operator:
     @waves_per_tile.setter
     def waves_per_tile(self, wt: int):
         check(lib().dgc_op_set_waves_per_tile(self._h, wt))
+
+    @property
+    def precision(self) -> str:
+        """'fp64' or 'fp32-mfma' (FP32 matrix-core contractions, FP64 state)."""
+        return "fp32-mfma" if lib().dgc_op_precision(self._h) == 1 else "fp64"
+
+    @precision.setter
+    def precision(self, value: str):
+        check(lib().dgc_op_set_precision(self._h, {"fp64": 0, "fp32-mfma": 1}[value]))
 
     def autotune(self, state, work, reps=20) -> dict:
         """Time every built tiling on these buffers and keep the fastest."""

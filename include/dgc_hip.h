@@ -125,6 +125,13 @@ int dgc_op_num_fields(dgc_op_t op);
  * wt = 1, 2, 4 (-1 where not built). No reference counterpart. */
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt);
 int dgc_op_waves_per_tile(dgc_op_t op);
+/* Arithmetic of the stage contractions (no reference counterpart).
+ * 0: FP64 throughout (default). 1: Dr u, Ds u and LIFT flux on the matrix
+ * cores in FP32 (16x16x4 f32 MFMA, operands rounded to FP32), with the state,
+ * stage accumulation, metric terms, fluxes and the stage update in FP64.
+ * Form 1 is built for the advection operator. */
+int dgc_op_set_precision(dgc_op_t op, int precision);
+int dgc_op_precision(dgc_op_t op);
 int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
                     float* ms_out);
 
