@@ -73,8 +73,22 @@ inline int mfma_operand_blocks(int Np, int Nfp) {
 }
 hipError_t launch_stage_mfma(int order, int stage, const StageParams& p, hipStream_t s);
 
+// Batched Maxwell TM ensemble (dg_ensemble.hip): one stage of every instance
+// of one order. params: 4 stage parameter records per instance (stages 1..4);
+// items: (instance, tile group) pairs of those instances, the items of an
+// instance contiguous.
+bool ensemble_supports(int order, int wt);
+hipError_t launch_ensemble_stage(int order, int wt, int stage, const StageParams* params, const int2* items,
+                                 int nitems, hipStream_t s);
+
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }
+
+// LDS (doubles) of one FP64 stage workgroup: node rows and face fluxes of its
+// 4/wt tiles.
+constexpr int stage_lds_doubles(int N, int physics, int wt) {
+  return (kWaves / wt) * (physics == kMaxwellTM ? 3 : 1) * ((N + 1) * (N + 2) / 2 + 3 * (N + 1)) * kTile;
+}
 
 // wt: waves per 64-cell tile (1, 2 or 4); a variant whose LDS would not fit
 // falls back to 4 (supported_waves_per_tile tells which one runs).

@@ -1,278 +1,13 @@
-// Fused nodal-DG right-hand side + classical RK4 stage update (FP64).
-//
-// Replaces, as one kernel per stage:
-//   examples/maxwell_2d/d_maxwell_2d.cuh:50-131   (Maxwell TM volume, upwind/PEC flux, LIFT)
-//   examples/advection_2d/d_advection_2d.cuh:45-100 (advection volume, upwind flux, LIFT)
-//   src/device_SemiDiscOp_wrapper.cuh:35-58        (grid-stride launcher)
-//   src/explicit_runge_kutta.h:33-51 + src/axpy.h  (6 axpy passes per step, fused here)
-//
-// Mapping. A tile is 64 consecutive cells (lane = cell) served by WT waves
-// (WT = 1, 2 or 4, chosen per order by the autotuner); a 256-thread
-// workgroup holds 4/WT tiles, and workgroups are renumbered so that each XCD
-// (own L2) sweeps a contiguous range of tiles, keeping neighbour traces in
-// the L2 that loaded them.
-//
-// Memory. Everything a tile needs from HBM is requested before the first
-// barrier: the connectivity codes, the node rows of the stage input, the four
-// metric terms (rx, ry, sx, sy) and the neighbour traces that live outside
-// the tile. Face normals and face_J/J are derived from the metric terms
-// (straight-sided cells), so a stage reads 4 doubles + 3 ints of geometry per
-// cell instead of 13 doubles. Node rows go to LDS; neighbours inside the tile
-// are read from LDS.
-//
-// Compute. Each wave owns ceil(Np/WT) output rows: their Dr/Ds/LIFT
-// coefficients are wave-uniform, so they arrive through scalar loads, and
-// each LDS read of a DOF feeds 2*rows FMAs. The RK4 stage update is applied
-// while the result is in registers (operand order of the reference's axpy).
-#include "dg_internal.hpp"
+// Per-order launchers of the fused FP64 stage (the device code is in
+// dg_stage_tile.hpp, shared with the ensemble kernel).
+#include "dg_stage_tile.hpp"
 
 namespace dgc {
 
-// Bijective workgroup renumbering: consecutive hardware dispatch slots go to
-// the 8 XCDs in turn; give XCD x the contiguous range of logical groups
-// [x*per + min(x, rem), ...).
-__device__ __forceinline__ int xcd_group(int b, int G) {
-  constexpr int X = 8;
-  const int x = b % X, i = b / X, per = G / X, rem = G % X;
-  return x * per + (x < rem ? x : rem) + i;
-}
-
 template <int N, int PHYS, int STAGE, int WT>
 __global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
-  using T = Tri<N>;
-  constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
-  constexpr int F = PHYS == kMaxwellTM ? 3 : 1;
-  constexpr int TPW = kWaves / WT;        // tiles per workgroup
-  constexpr int RPW = (Np + WT - 1) / WT;  // output rows per wave (contiguous block)
-  constexpr int LPW = (Np + WT - 1) / WT;  // input rows loaded per wave (strided)
-  constexpr int FPW = (NF + WT - 1) / WT;  // face slots per wave (strided)
-  constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
-  __shared__ double us[TPW][F][Np][kTile];
-  __shared__ double fl[TPW][F][NF][kTile];
-
-  const int K = p.K;
-  const int lane = threadIdx.x & (kTile - 1);
-  const int wave = uniform(threadIdx.x >> 6);
-  const int tw = wave / WT;
-  const int wt = wave - tw * WT;
-  const int c0 = (xcd_group(blockIdx.x, gridDim.x) * TPW + tw) * kTile;
-  const int c = c0 + lane;
-  const bool valid = c < K;
-
-  double t = p.t;
-  if (p.tclock != nullptr) t = p.tclock[0] + p.tfrac * p.dt_clock;
-
-  // ---- 1. issue every independent HBM read of the tile --------------------
-  int code[FPW];
-#pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
-    code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
-  }
-  double tl[F][LPW];
-#pragma unroll
-  for (int i = 0; i < LPW; ++i) {
-    const int j = wt + WT * i;
-#pragma unroll
-    for (int f = 0; f < F; ++f) tl[f][i] = (valid && j < Np) ? p.in[f][(size_t)j * K + c] : 0.0;
-  }
-  double rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0;
-  if (valid) {
-    rx = p.geo[(size_t)kRx * K + c];
-    ry = p.geo[(size_t)kRy * K + c];
-    sx = p.geo[(size_t)kSx * K + c];
-    sy = p.geo[(size_t)kSy * K + c];
-  }
-  double gv[F][FPW];  // neighbour traces outside the tile
-#pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
-    const int e = m / Nfp, d = m - e * Nfp;
-    const int nf = code[i] & 3, idx = code[i] >> 2;
-    const bool far = nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile;
-    const int nn = T::fnode(nf & 3, Nfp - 1 - d);
-    (void)e;
-#pragma unroll
-    for (int f = 0; f < F; ++f) gv[f][i] = far ? p.in[f][(size_t)nn * K + idx] : 0.0;
-  }
-#pragma unroll
-  for (int i = 0; i < LPW; ++i) {
-    const int j = wt + WT * i;
-    if (j < Np) {
-#pragma unroll
-      for (int f = 0; f < F; ++f) us[tw][f][j][lane] = tl[f][i];
-    }
-  }
-  __syncthreads();
-
-  // ---- 2. numerical fluxes at the face nodes, scaled by face_J / J ---------
-#pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
-    if (m >= NF) break;
-    const int e = m / Nfp, d = m - e * Nfp;
-    const int own = T::fnode(e, d);
-    double a[F], b[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) a[f] = us[tw][f][own][lane];
-    // outward normal and face_J/J of face e from the metric terms
-    const double vx = e == 0 ? sx : (e == 1 ? rx + sx : rx);
-    const double vy = e == 0 ? sy : (e == 1 ? ry + sy : ry);
-    const double fs = sqrt(vx * vx + vy * vy);
-    const double sg = (e == 1 ? 1.0 : -1.0) / (valid ? fs : 1.0);
-    const double nx = sg * vx, ny = sg * vy;
-    const int nf = code[i] & 3, idx = code[i] >> 2;
-    if (nf == kBoundaryCode) {
-      if (PHYS == kMaxwellTM) {
-        // PEC wall: (Hx, Hy, Ez)+ = (Hx, Hy, -Ez)-
-        b[0] = a[0];
-        if (F == 3) {
-          b[1 % F] = a[1 % F];
-          b[2 % F] = -a[2 % F];
-        }
-      } else if (p.bc == kBcSinSin && valid) {
-        const double xb = p.bx[idx + d], yb = p.by[idx + d];
-        b[0] = sin((xb + 1.0 - t) * M_PI) * sin((yb + 1.0 - t) * M_PI);
-      } else {
-        b[0] = 0.0;
-      }
-    } else {
-      const int nn = T::fnode(nf, Nfp - 1 - d);
-      const int l = idx - c0;
-      if ((unsigned)l < (unsigned)kTile) {
-#pragma unroll
-        for (int f = 0; f < F; ++f) b[f] = us[tw][f][nn][l];
-      } else {
-#pragma unroll
-        for (int f = 0; f < F; ++f) b[f] = gv[f][i];
-      }
-    }
-    if (PHYS == kMaxwellTM) {
-      const double dHx = a[0] - b[0];
-      const double dHy = a[1 % F] - b[1 % F];
-      const double sEz = a[2 % F] + b[2 % F];
-      const double w = sEz + ny * dHx - nx * dHy;
-      fl[tw][0][m][lane] = 0.5 * ny * w * fs;
-      if (F == 3) {
-        fl[tw][1 % F][m][lane] = -0.5 * nx * w * fs;
-        fl[tw][2 % F][m][lane] =
-            0.5 * (ny * (a[0] + b[0]) - nx * (a[1 % F] + b[1 % F]) + a[2 % F] - b[2 % F]) * fs;
-      }
-    } else {
-      const double an = p.ax * nx + p.ay * ny;
-      const double U = an >= 0.0 ? a[0] : b[0];
-      fl[tw][0][m][lane] = U * an * fs;
-    }
-  }
-  __syncthreads();
-
-  // ---- 3. stage operands of this wave's rows (in flight during the FMAs) --
-  const int j0 = wt * RPW;
-  double o1[F][RPW], o2[F][RPW], o3[F][RPW];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    const int j = j0 + i;
-    const bool ok = valid && j < Np;
-    const size_t idx = (size_t)j * K + c;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      o1[f][i] = o2[f][i] = o3[f][i] = 0.0;
-      if (ok) {
-        if (STAGE == 2) {
-          o1[f][i] = p.acc[f][idx];
-          o2[f][i] = p.u[f][idx];
-        } else if (STAGE == 3) {
-          o2[f][i] = p.u[f][idx];
-        } else if (STAGE == 4) {
-          o1[f][i] = p.acc[f][idx];
-          o2[f][i] = p.u[f][idx];
-          o3[f][i] = p.k3[f][idx];
-        }
-      }
-    }
-  }
-
-  // ---- 4. volume term: Dr.u and Ds.u for this wave's rows -------------------
-  double rr[F][RPW], ss[F][RPW];
-#pragma unroll
-  for (int f = 0; f < F; ++f)
-#pragma unroll
-    for (int i = 0; i < RPW; ++i) rr[f][i] = ss[f][i] = 0.0;
-#pragma unroll
-  for (int k = 0; k < Np; ++k) {
-    double uk[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) uk[f] = us[tw][f][k][lane];
-    const double* dr = p.DrT + k * S + j0;
-    const double* ds = p.DsT + k * S + j0;
-#pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const double cr = dr[i], cs = ds[i];
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        rr[f][i] = fma(cr, uk[f], rr[f][i]);
-        ss[f][i] = fma(cs, uk[f], ss[f][i]);
-      }
-    }
-  }
-  double res[F][RPW];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    if (PHYS == kMaxwellTM) {
-      // Hx_t = Dy Ez, Hy_t = -Dx Ez, Ez_t = Dy Hx - Dx Hy
-      res[0][i] = ry * rr[2 % F][i] + sy * ss[2 % F][i];
-      if (F == 3) {
-        res[1 % F][i] = -(rx * rr[2 % F][i] + sx * ss[2 % F][i]);
-        res[2 % F][i] = (ry * rr[0][i] + sy * ss[0][i]) - (rx * rr[1 % F][i] + sx * ss[1 % F][i]);
-      }
-    } else {
-      res[0][i] = (p.ax * rx + p.ay * ry) * rr[0][i] + (p.ax * sx + p.ay * sy) * ss[0][i];
-    }
-  }
-
-  // ---- 5. lift: res -= LIFT . (flux * face_J / J) ----------------------------
-#pragma unroll
-  for (int m = 0; m < NF; ++m) {
-    double fm[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) fm[f] = fl[tw][f][m][lane];
-    const double* lt = p.LT + m * S + j0;
-#pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const double cl = lt[i];
-#pragma unroll
-      for (int f = 0; f < F; ++f) res[f][i] = fma(-cl, fm[f], res[f][i]);
-    }
-  }
-
-  // ---- 6. stage update (reference operand order: out = a * x + y) ------------
-  if (!valid) return;
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    const int j = j0 + i;
-    if (j >= Np) break;
-    const size_t idx = (size_t)j * K + c;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      const double r = res[f][i];
-      if (STAGE == 0) {
-        p.out[f][idx] = r;
-      } else if (STAGE == 1) {
-        p.acc[f][idx] = r;
-        p.out[f][idx] = p.c1 * r + us[tw][f][j][lane];
-      } else if (STAGE == 2) {
-        p.acc[f][idx] = 2.0 * r + o1[f][i];
-        p.out[f][idx] = p.c1 * r + o2[f][i];
-      } else if (STAGE == 3) {
-        p.k3[f][idx] = r;
-        p.out[f][idx] = p.c1 * r + o2[f][i];
-      } else {
-        const double w1 = 2.0 * o3[f][i] + r;
-        const double w2 = p.c1 * o1[f][i] + o2[f][i];
-        p.u[f][idx] = p.c1 * w1 + w2;
-      }
-    }
-  }
+  __shared__ double lds[stage_lds_doubles(N, PHYS, WT)];
+  stage_tile<N, PHYS, STAGE, WT>(p, xcd_group(blockIdx.x, gridDim.x), lds);
 }
 
 namespace {

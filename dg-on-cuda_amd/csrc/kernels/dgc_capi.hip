@@ -2,6 +2,7 @@
 // semi-discrete operators and the RK4 time loop. See include/dgc_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -706,6 +707,250 @@ int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, doubl
   }
   if (t_out) *t_out = t;
   if (steps_out) *steps_out = steps;
+  return DGC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensemble of independent Maxwell TM instances.
+struct EnsembleGroup {
+  int order = 0, wt = 4, offset = 0, nitems = 0;
+  hipStream_t stream = nullptr;  // branch of the step graph
+};
+
+struct dgc_ensemble_s {
+  dgc_ctx_s* ctx = nullptr;
+  std::vector<dgc_op_s*> ops;
+  std::vector<std::vector<double*>> state, work;  // per instance: F and 4F device vectors
+  std::vector<double> dt;
+  std::vector<int> wt;
+  std::vector<EnsembleGroup> groups;  // one per order present
+  dgc::StageParams* d_params = nullptr;  // [instance][stage 1..4]
+  int2* d_items = nullptr;
+  int nitems = 0;
+  hipGraphExec_t graph = nullptr;
+};
+
+static void ensemble_release(dgc_ensemble_s* e) {
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  e->graph = nullptr;
+  for (auto& g : e->groups)
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+  e->groups.clear();
+  (void)hipFree(e->d_params);
+  (void)hipFree(e->d_items);
+  e->d_params = nullptr;
+  e->d_items = nullptr;
+}
+
+// Work lists for the current tilings: per order, the instances by decreasing
+// size, each as contiguous (instance, tile group) items; and the stage
+// parameters of every instance (the buffers of dgc_rk4_step's stages).
+static int ensemble_build(dgc_ensemble_s* e) {
+  (void)hipStreamSynchronize(e->ctx->stream);
+  ensemble_release(e);
+  const int n = int(e->ops.size());
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    const dgc_disc_s *da = e->ops[a]->disc, *db = e->ops[b]->disc;
+    return da->order != db->order ? da->order > db->order : da->K > db->K;
+  });
+  std::vector<dgc::StageParams> params(4 * size_t(n));
+  for (int i = 0; i < n; ++i) {
+    const double h = e->dt[i];
+    const double c1[4] = {0.5 * h, 0.5 * h, h, h / 6.0};
+    const int in_buf[4] = {-1, 1, 2, 1}, out_buf[4] = {1, 2, 1, -1};
+    for (int s = 0; s < 4; ++s) {
+      dgc::StageParams p = base_params(e->ops[i]);
+      p.c1 = c1[s];
+      p.opF = nullptr;
+      for (int f = 0; f < 3; ++f) {
+        p.u[f] = e->state[i][f];
+        p.acc[f] = e->work[i][0 * 3 + f];
+        p.k3[f] = e->work[i][3 * 3 + f];
+        p.in[f] = in_buf[s] < 0 ? e->state[i][f] : e->work[i][in_buf[s] * 3 + f];
+        p.out[f] = out_buf[s] < 0 ? e->state[i][f] : e->work[i][out_buf[s] * 3 + f];
+      }
+      params[4 * size_t(i) + s] = p;
+    }
+  }
+  std::vector<int2> items;
+  for (int i : idx) {
+    const int order = e->ops[i]->disc->order;
+    if (e->groups.empty() || e->groups.back().order != order) {
+      EnsembleGroup g;
+      g.order = order;
+      g.wt = e->wt[i];  // the largest instance of the order sets the group's tiling
+      g.offset = int(items.size());
+      e->groups.push_back(g);
+    }
+    EnsembleGroup& g = e->groups.back();
+    e->wt[i] = g.wt;
+    const int cells = dgc::kTile * (dgc::kWaves / g.wt);
+    const int ngroups = (e->ops[i]->disc->K + cells - 1) / cells;
+    for (int t = 0; t < ngroups; ++t) items.push_back(make_int2(i, t));
+    g.nitems = int(items.size()) - g.offset;
+  }
+  for (auto& g : e->groups) DGC_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  e->nitems = int(items.size());
+  DGC_HIP_TRY(hipMalloc(&e->d_params, params.size() * sizeof(params[0])));
+  DGC_HIP_TRY(hipMalloc(&e->d_items, items.size() * sizeof(int2)));
+  DGC_HIP_TRY(hipMemcpy(e->d_params, params.data(), params.size() * sizeof(params[0]), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(e->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return DGC_OK;
+}
+
+static int supported_ensemble_wt(int order, int wt) { return dgc::ensemble_supports(order, wt) ? wt : 4; }
+
+extern "C" {
+
+int dgc_ensemble_create(dgc_ctx_t ctx, int n, const dgc_op_t* ops, double* const* states, double* const* work,
+                        const double* dt, dgc_ensemble_t* out) {
+  DGC_REQUIRE(ctx && ops && states && work && dt && out && n >= 1, "dgc_ensemble_create: bad argument");
+  auto* e = new dgc_ensemble_s;
+  e->ctx = ctx;
+  for (int i = 0; i < n; ++i) {
+    dgc_op_s* op = ops[i];
+    bool ok = op && op->physics == dgc::kMaxwellTM && op->disc->ctx == ctx && op->precision == 0 &&
+              std::isfinite(dt[i]) && dt[i] > 0.0;
+    for (int f = 0; ok && f < 3; ++f) ok = states[3 * i + f] != nullptr;
+    for (int w = 0; ok && w < 12; ++w) ok = work[12 * i + w] != nullptr;
+    if (!ok) {
+      delete e;
+      return dgc_set_error(DGC_ERR_INVALID, "dgc_ensemble_create: instance " + std::to_string(i) +
+                                                " needs a FP64 Maxwell TM operator on this context, dt > 0, "
+                                                "3 state and 12 work vectors");
+    }
+    e->ops.push_back(op);
+    e->state.emplace_back(states + 3 * i, states + 3 * i + 3);
+    e->work.emplace_back(work + 12 * i, work + 12 * i + 12);
+    e->dt.push_back(dt[i]);
+    e->wt.push_back(supported_ensemble_wt(op->disc->order, op->wt));
+  }
+  int rc = ensemble_build(e);
+  if (rc != DGC_OK) {
+    ensemble_release(e);
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return DGC_OK;
+}
+
+int dgc_ensemble_destroy(dgc_ensemble_t e) {
+  if (!e) return DGC_OK;
+  (void)hipStreamSynchronize(e->ctx->stream);
+  ensemble_release(e);
+  delete e;
+  return DGC_OK;
+}
+
+int dgc_ensemble_autotune(dgc_ensemble_t e, int reps) {
+  DGC_REQUIRE(e && reps > 0, "dgc_ensemble_autotune: bad argument");
+  const int n = int(e->ops.size());
+  for (int order = 1; order <= dgc::kMaxOrder; ++order) {
+    int big = -1;
+    for (int i = 0; i < n; ++i)
+      if (e->ops[i]->disc->order == order && (big < 0 || e->ops[i]->disc->K > e->ops[big]->disc->K)) big = i;
+    if (big < 0) continue;
+    dgc_op_s* op = e->ops[big];
+    float ms[3];
+    int rc = dgc_op_autotune(op, e->state[big].data(), size_t(op->disc->K) * op->disc->Np, e->work[big].data(),
+                             reps, ms);
+    if (rc != DGC_OK) return rc;
+    for (int i = 0; i < n; ++i)
+      if (e->ops[i]->disc->order == order) e->wt[i] = supported_ensemble_wt(order, op->wt);
+  }
+  return ensemble_build(e);
+}
+
+int dgc_ensemble_info(dgc_ensemble_t e, int* wt, int* num_items, int* grid, size_t* lds_bytes) {
+  DGC_REQUIRE(e, "dgc_ensemble_info: null ensemble");
+  if (wt) std::copy(e->wt.begin(), e->wt.end(), wt);
+  if (num_items) *num_items = e->nitems;
+  int g = 0;
+  size_t lds = 0;
+  for (const auto& gr : e->groups) {
+    g += gr.nitems;
+    lds = std::max(lds, size_t(dgc::stage_lds_doubles(gr.order, dgc::kMaxwellTM, gr.wt)) * sizeof(double));
+  }
+  if (grid) *grid = g;
+  if (lds_bytes) *lds_bytes = lds;
+  return DGC_OK;
+}
+
+int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms) {
+  DGC_REQUIRE(e && nsteps >= 0, "dgc_ensemble_steps: bad argument");
+  hipStream_t st = e->ctx->stream;
+  if (!e->graph) {
+    // one step: fork one branch per order, 4 stage launches each, join
+    hipGraph_t g = nullptr;
+    std::vector<hipEvent_t> evs(e->groups.size() + 1, nullptr);
+    hipError_t le = hipSuccess;
+    for (auto& ev : evs)
+      if (le == hipSuccess) le = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (le == hipSuccess) le = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    const bool capturing = le == hipSuccess;
+    if (le == hipSuccess) le = hipEventRecord(evs[0], st);
+    for (size_t k = 0; k < e->groups.size() && le == hipSuccess; ++k) {
+      const EnsembleGroup& gr = e->groups[k];
+      le = hipStreamWaitEvent(gr.stream, evs[0], 0);
+      for (int s = 1; s <= 4 && le == hipSuccess; ++s)
+        le = dgc::launch_ensemble_stage(gr.order, gr.wt, s, e->d_params, e->d_items + gr.offset, gr.nitems,
+                                        gr.stream);
+      if (le == hipSuccess) le = hipEventRecord(evs[k + 1], gr.stream);
+      if (le == hipSuccess) le = hipStreamWaitEvent(st, evs[k + 1], 0);
+    }
+    hipError_t ce = capturing ? hipStreamEndCapture(st, &g) : hipSuccess;
+    for (auto& ev : evs)
+      if (ev) (void)hipEventDestroy(ev);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return dgc_set_error(DGC_ERR_LAUNCH, std::string("dgc_ensemble_steps: capture: ") +
+                                               hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    ce = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ce != hipSuccess) {
+      e->graph = nullptr;
+      return dgc_set_error(DGC_ERR_HIP, std::string("dgc_ensemble_steps: instantiate: ") + hipGetErrorString(ce));
+    }
+  }
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventRecord(e0, st));
+  for (int i = 0; i < nsteps; ++i) DGC_HIP_TRY(hipGraphLaunch(e->graph, st));
+  DGC_HIP_TRY(hipEventRecord(e1, st));
+  DGC_HIP_TRY(hipEventSynchronize(e1));
+  float el = 0.f;
+  (void)hipEventElapsedTime(&el, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = el;
+  return DGC_OK;
+}
+
+int dgc_ensemble_steps_sequential(dgc_ensemble_t e, int nsteps, float* ms) {
+  DGC_REQUIRE(e && nsteps >= 0, "dgc_ensemble_steps_sequential: bad argument");
+  hipStream_t st = e->ctx->stream;
+  hipEvent_t e0, e1;
+  DGC_HIP_TRY(hipEventCreate(&e0));
+  DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventRecord(e0, st));
+  for (size_t i = 0; i < e->ops.size(); ++i) {
+    int rc = run_full_steps(e->ops[i], e->state[i].data(), 0.0, e->dt[i], nsteps, e->work[i].data());
+    if (rc != DGC_OK) return rc;
+  }
+  DGC_HIP_TRY(hipEventRecord(e1, st));
+  DGC_HIP_TRY(hipEventSynchronize(e1));
+  float el = 0.f;
+  (void)hipEventElapsedTime(&el, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = el;
   return DGC_OK;
 }
 

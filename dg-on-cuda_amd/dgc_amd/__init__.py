@@ -20,3 +20,4 @@ from .core import (  # noqa: F401
 )
 from .ipdg import IPDG, poisson_rhs  # noqa: F401
 from .mpnn import LearnedPreconditioner  # noqa: F401
+from .ensemble import Ensemble  # noqa: F401

@@ -290,6 +290,32 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
 int dgc_rk4_profile_stages(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
                            int nsteps, double* const* wk, float* stage_ms);
 
+/* ---- batched ensemble (no reference counterpart: the reference runs one
+ * mesh per process, examples/maxwell_2d/main.cpp) ----------------------------
+ * n independent FP64 Maxwell TM instances (each its own discretization, size
+ * and order) advanced together: one RK4 stage of all instances of one order
+ * is ONE kernel launch over a work list of (instance, tile group) items; a
+ * step (4 stages per order present, the orders as independent branches) is
+ * replayed from a HIP graph. Instance i
+ * uses operator ops[i], state vectors states[3i..3i+2] (Hx, Hy, Ez), work
+ * vectors work[12i..12i+11] (4 groups of 3, as dgc_rk4_step's wk) and time
+ * step dt[i]; all device vectors stay owned by the caller. */
+typedef struct dgc_ensemble_s* dgc_ensemble_t;
+int dgc_ensemble_create(dgc_ctx_t ctx, int n, const dgc_op_t* ops, double* const* states, double* const* work,
+                        const double* dt, dgc_ensemble_t* out);
+int dgc_ensemble_destroy(dgc_ensemble_t e);
+/* Picks the tiling (64, 128 or 256 cells per workgroup) of each order present
+ * by timing that order's largest instance (dgc_op_autotune; only work vectors
+ * are written) and rebuilds the work list. */
+int dgc_ensemble_autotune(dgc_ensemble_t e, int reps);
+/* wt[n]: waves per 64-cell tile per instance; work-list length, launch grid, LDS bytes. */
+int dgc_ensemble_info(dgc_ensemble_t e, int* wt, int* num_items, int* grid, size_t* lds_bytes);
+/* nsteps RK4 steps of every instance (graph replays); *ms: HIP-event time. */
+int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms);
+/* Baseline: the same steps instance after instance with the per-instance
+ * kernels and step graphs (dgc_rk4_steps); *ms: HIP-event time. */
+int dgc_ensemble_steps_sequential(dgc_ensemble_t e, int nsteps, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
