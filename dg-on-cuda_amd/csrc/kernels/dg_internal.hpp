@@ -59,10 +59,15 @@ struct StageParams {
   double* u[3];
   double* acc[3];
   double* k3[3];
-  // FP32 matrix-core operand blocks of Dr, Ds and LIFT (advection only): when
-  // set, the stage runs the FP32-MFMA contraction kernel (dg_rk_mfma.hip).
+  // Matrix-core operand blocks of Dr, Ds and LIFT (advection only,
+  // dg_rk_mfma.hip): with opF set the stage runs the FP32-contraction form;
+  // the FP64 blocks opD serve the FP64 matrix-core variant (tiling code 0).
   const float* __restrict__ opF;
+  const double* __restrict__ opD;
 };
+
+// Tiling code of the FP64 matrix-core variant of the advection stage.
+constexpr int kMfmaTiling = 0;
 
 // Number of FP32 operand blocks (16 rows x 4 columns, 64 floats each) of the
 // MFMA stage kernel: Dr and Ds as ceil(Np/16) x ceil(Np/4) blocks, LIFT as

@@ -6,6 +6,9 @@
 // v_mfma_f32_16x16x4f32       D[16x16] += A[16x4] * B[4x16]   (fp32)
 //   lane l holds A[l%16][l/16] and B[l/16][l%16]; D register r of lane l is
 //   D[4*(l/16) + r][l%16].
+// v_mfma_f64_16x16x4f64       the same shapes and A/B layout in fp64, but the
+//   result is row-interleaved: D register pair r of lane l is
+//   D[4*r + l/16][l%16].
 // dgc_selftest_mfma (dg_selftest.hip) checks both layouts on the device.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -40,6 +43,13 @@ __device__ __forceinline__ floatx16 mfma32x32x8bf16(bf16x4 a, bf16x4 b, floatx16
 
 __device__ __forceinline__ floatx4 mfma16x16x4f32(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f64_16x16x4f64: the fp32 16x16x4 operand layout, rows of the result
+// interleaved (register r of lane l holds row 4 r + l / 16)
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ doublex4 mfma16x16x4f64(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
 }  // namespace dgc

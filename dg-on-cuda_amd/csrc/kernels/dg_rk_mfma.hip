@@ -1,19 +1,21 @@
 // Advection RK4 stage with the volume and lift contractions on the matrix
-// cores in FP32 (v_mfma_f32_16x16x4f32); the state, the stage accumulation,
-// the metric terms, the numerical fluxes and the stage update stay FP64.
+// cores, in FP32 (v_mfma_f32_16x16x4f32: the FP32-contraction form) or in
+// FP64 (v_mfma_f64_16x16x4f64: a variant of the plain FP64 path that the
+// autotuner can pick). The state, the stage accumulation, the metric terms,
+// the numerical fluxes and the stage update are FP64 in both.
 //
 // Same contract as the FP64 stage kernel (dg_rk.hip; it replaces
 // examples/advection_2d/d_advection_2d.cuh:45-100 and the axpy passes of
 // src/explicit_runge_kutta.h:33-51). Per 64-cell tile and 256-thread
 // workgroup:
 //   1. every HBM read of the tile is issued first (node rows, connectivity,
-//      metric terms, neighbour traces outside the tile); the FP32 operand
-//      blocks are copied to LDS;
+//      metric terms, neighbour traces outside the tile); the operand blocks
+//      are copied to LDS;
 //   2. upwind fluxes at the 3 Nfp face nodes in FP64 (LDS);
 //   3. wave w owns cells 16w..16w+15 of the tile as the N-dimension of
 //      D[16 x 16] += A[16 x 4] B[4 x 16] products: A = 16-row blocks of Dr,
-//      Ds and LIFT (FP32), B = node values / fluxes rounded to FP32. The
-//      lane holding output row j of cell q combines
+//      Ds and LIFT, B = node values / fluxes (rounded to FP32 in the FP32
+//      form). The lane holding output row j of cell q combines
 //         (a_x r_x + a_y r_y)(Dr u)_j + (a_x s_x + a_y s_y)(Ds u)_j - (LIFT f)_j
 //      in FP64 and applies the RK4 stage update in FP64.
 #include "dg_internal.hpp"
@@ -31,7 +33,36 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * per + (x < rem ? x : rem) + i;
 }
 
-template <int N, int STAGE>
+__device__ __forceinline__ floatx4 mma(float a, float b, floatx4 c) { return mfma16x16x4f32(a, b, c); }
+__device__ __forceinline__ doublex4 mma(double a, double b, doublex4 c) { return mfma16x16x4f64(a, b, c); }
+
+// Result type and row of result register r in lane group kq = lane / 16:
+// fp32 rows are blocked (4 kq + r), fp64 rows interleaved (4 r + kq).
+template <typename CT>
+struct Acc;
+template <>
+struct Acc<float> {
+  using type = floatx4;
+  static constexpr __device__ int row(int kq, int r) { return 4 * kq + r; }
+};
+template <>
+struct Acc<double> {
+  using type = doublex4;
+  static constexpr __device__ int row(int kq, int r) { return 4 * r + kq; }
+};
+
+template <typename CT>
+__device__ __forceinline__ const CT* operand_blocks(const StageParams& p);
+template <>
+__device__ __forceinline__ const float* operand_blocks<float>(const StageParams& p) {
+  return p.opF;
+}
+template <>
+__device__ __forceinline__ const double* operand_blocks<double>(const StageParams& p) {
+  return p.opD;
+}
+
+template <typename CT, int N, int STAGE>
 __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams p) {
   using T = Tri<N>;
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
@@ -41,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
   constexpr int FPW = (NF + kWaves - 1) / kWaves;
   __shared__ double us[Np][kTile];
   __shared__ double fl[NF][kTile];
-  __shared__ float opa[NA][64];
+  __shared__ CT opa[NA][64];
 
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
@@ -81,7 +112,10 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
     const bool far = nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile;
     gv[i] = far ? p.in[0][(size_t)T::fnode(nf & 3, Nfp - 1 - d) * K + idx] : 0.0;
   }
-  for (int i = threadIdx.x; i < NA * 64; i += kBlock) (&opa[0][0])[i] = p.opF[i];
+  {
+    const CT* src = operand_blocks<CT>(p);
+    for (int i = threadIdx.x; i < NA * 64; i += kBlock) (&opa[0][0])[i] = src[i];
+  }
 #pragma unroll
   for (int i = 0; i < LPW; ++i) {
     const int j = wave + kWaves * i;
@@ -129,16 +163,16 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
     alpha = p.ax * p.geo[(size_t)kRx * K + cc] + p.ay * p.geo[(size_t)kRy * K + cc];
     beta = p.ax * p.geo[(size_t)kSx * K + cc] + p.ay * p.geo[(size_t)kSy * K + cc];
   }
-  float bu[KS], bl[KL];
+  CT bu[KS], bl[KL];
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
     const int k = 4 * ks + kq;
-    bu[ks] = k < Np ? float(us[k][q]) : 0.f;
+    bu[ks] = k < Np ? CT(us[k][q]) : CT(0);
   }
 #pragma unroll
   for (int ks = 0; ks < KL; ++ks) {
     const int m = 4 * ks + kq;
-    bl[ks] = m < NF ? float(fl[m][q]) : 0.f;
+    bl[ks] = m < NF ? CT(fl[m][q]) : CT(0);
   }
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) {
@@ -146,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
     double o1[4], o2[4], o3[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int j = 16 * mb + 4 * kq + r;
+      const int j = 16 * mb + Acc<CT>::row(kq, r);
       const size_t idx = (size_t)j * K + cc;
       o1[r] = o2[r] = o3[r] = 0.0;
       if (ovalid && j < Np) {
@@ -162,17 +196,17 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
         }
       }
     }
-    floatx4 ar = {0.f, 0.f, 0.f, 0.f}, as = ar, al = ar;
+    typename Acc<CT>::type ar = {0, 0, 0, 0}, as = ar, al = ar;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      ar = mfma16x16x4f32(opa[mb * KS + ks][lane], bu[ks], ar);
-      as = mfma16x16x4f32(opa[MB * KS + mb * KS + ks][lane], bu[ks], as);
+      ar = mma(opa[mb * KS + ks][lane], bu[ks], ar);
+      as = mma(opa[MB * KS + mb * KS + ks][lane], bu[ks], as);
     }
 #pragma unroll
-    for (int ks = 0; ks < KL; ++ks) al = mfma16x16x4f32(opa[2 * MB * KS + mb * KL + ks][lane], bl[ks], al);
+    for (int ks = 0; ks < KL; ++ks) al = mma(opa[2 * MB * KS + mb * KL + ks][lane], bl[ks], al);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int j = 16 * mb + 4 * kq + r;
+      const int j = 16 * mb + Acc<CT>::row(kq, r);
       if (!ovalid || j >= Np) continue;
       const double res = alpha * double(ar[r]) + beta * double(as[r]) - double(al[r]);
       const size_t idx = (size_t)j * K + cc;
@@ -198,30 +232,37 @@ __global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams
 
 using StageFn = void (*)(const StageParams);
 
-template <int N>
+template <typename CT, int N>
 StageFn mfma_fn(int stage) {
-  return stage == 0 ? dg_stage_mfma_kernel<N, 0>
-       : stage == 1 ? dg_stage_mfma_kernel<N, 1>
-       : stage == 2 ? dg_stage_mfma_kernel<N, 2>
-       : stage == 3 ? dg_stage_mfma_kernel<N, 3>
-                    : dg_stage_mfma_kernel<N, 4>;
+  return stage == 0 ? dg_stage_mfma_kernel<CT, N, 0>
+       : stage == 1 ? dg_stage_mfma_kernel<CT, N, 1>
+       : stage == 2 ? dg_stage_mfma_kernel<CT, N, 2>
+       : stage == 3 ? dg_stage_mfma_kernel<CT, N, 3>
+                    : dg_stage_mfma_kernel<CT, N, 4>;
+}
+
+template <typename CT>
+StageFn pick_mfma(int order, int stage) {
+  switch (order) {
+    case 1: return mfma_fn<CT, 1>(stage);
+    case 2: return mfma_fn<CT, 2>(stage);
+    case 3: return mfma_fn<CT, 3>(stage);
+    case 4: return mfma_fn<CT, 4>(stage);
+    case 5: return mfma_fn<CT, 5>(stage);
+    case 6: return mfma_fn<CT, 6>(stage);
+    case 7: return mfma_fn<CT, 7>(stage);
+    default: return nullptr;
+  }
 }
 
 }  // namespace
 
 hipError_t launch_stage_mfma(int order, int stage, const StageParams& p, hipStream_t s) {
-  if (stage < 0 || stage > 4 || p.opF == nullptr) return hipErrorInvalidValue;
-  StageFn fn = nullptr;
-  switch (order) {
-    case 1: fn = mfma_fn<1>(stage); break;
-    case 2: fn = mfma_fn<2>(stage); break;
-    case 3: fn = mfma_fn<3>(stage); break;
-    case 4: fn = mfma_fn<4>(stage); break;
-    case 5: fn = mfma_fn<5>(stage); break;
-    case 6: fn = mfma_fn<6>(stage); break;
-    case 7: fn = mfma_fn<7>(stage); break;
-    default: return hipErrorInvalidValue;
-  }
+  if (stage < 0 || stage > 4) return hipErrorInvalidValue;
+  // the FP32 blocks select the FP32-contraction form; else the FP64 variant
+  StageFn fn = p.opF != nullptr ? pick_mfma<float>(order, stage)
+             : p.opD != nullptr ? pick_mfma<double>(order, stage) : nullptr;
+  if (fn == nullptr) return hipErrorInvalidValue;
   const int grid = (p.K + kTile - 1) / kTile;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, s, p);
   return hipGetLastError();

@@ -30,10 +30,20 @@ __global__ void mfma_f32_probe(const float* A, const float* B, float* C) {
   for (int r = 0; r < 4; ++r) C[(4 * (l / 16) + r) * 16 + l % 16] = d[r];
 }
 
+__global__ void mfma_f64_probe(const float* A, const float* B, float* C) {
+  // A: 16x4, B: 4x16 row-major; products and sums in fp64. C receives the raw
+  // result registers, C[4 l + r] = register r of lane l, for the caller to
+  // match against the product.
+  const int l = threadIdx.x;
+  dgc::doublex4 d = {0.0, 0.0, 0.0, 0.0};
+  d = dgc::mfma16x16x4f64(double(A[(l % 16) * 4 + l / 16]), double(B[(l / 16) * 16 + l % 16]), d);
+  for (int r = 0; r < 4; ++r) C[4 * l + r] = float(d[r]);
+}
+
 }  // namespace
 
 extern "C" int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C) {
-  if (!ctx || !A || !B || !C || (kind != 0 && kind != 1))
+  if (!ctx || !A || !B || !C || kind < 0 || kind > 2)
     return dgc_set_error(DGC_ERR_INVALID, "dgc_selftest_mfma: bad argument");
   const int na = kind == 0 ? 32 * 8 : 16 * 4, nb = kind == 0 ? 8 * 32 : 4 * 16, nc = kind == 0 ? 32 * 32 : 16 * 16;
   float *dA = nullptr, *dB = nullptr, *dC = nullptr;
@@ -46,8 +56,10 @@ extern "C" int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const 
   if (e == hipSuccess) {
     if (kind == 0)
       hipLaunchKernelGGL(mfma_bf16_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
-    else
+    else if (kind == 1)
       hipLaunchKernelGGL(mfma_f32_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    else
+      hipLaunchKernelGGL(mfma_f64_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -44,7 +44,8 @@ struct dgc_op_s {
   int bc = 0;
   int wt = 4;  // waves per 64-cell tile of the stage kernels
   int precision = 0;      // 0: FP64; 1: FP32 matrix-core contractions (advection)
-  float* opF = nullptr;   // MFMA operand blocks of Dr, Ds, LIFT (built on first use)
+  float* opF = nullptr;   // matrix-core operand blocks of Dr, Ds, LIFT (advection)
+  double* opD = nullptr;
   std::vector<double> hDr, hDs, hL;  // host copies of the operators
   double* tclock = nullptr;
   // cached graph of one full RK4 step
@@ -281,11 +282,11 @@ static int build_mfma_operands(dgc_op_s* op) {
   const int Np = op->disc->Np, NF = 3 * op->disc->Nfp;
   const int MB = (Np + 15) / 16, KS = (Np + 3) / 4, KL = (NF + 3) / 4;
   const int NA = dgc::mfma_operand_blocks(Np, op->disc->Nfp);
-  std::vector<float> h(size_t(NA) * 64, 0.f);
+  std::vector<double> h(size_t(NA) * 64, 0.0);
   auto put = [&](int a, const std::vector<double>& M, int cols, int mb, int ks) {
     for (int l = 0; l < 64; ++l) {
       const int i = 16 * mb + l % 16, k = 4 * ks + l / 16;
-      if (i < Np && k < cols) h[size_t(a) * 64 + l] = float(M[size_t(i) * cols + k]);
+      if (i < Np && k < cols) h[size_t(a) * 64 + l] = M[size_t(i) * cols + k];
     }
   };
   for (int mb = 0; mb < MB; ++mb)
@@ -295,14 +296,18 @@ static int build_mfma_operands(dgc_op_s* op) {
     }
   for (int mb = 0; mb < MB; ++mb)
     for (int ks = 0; ks < KL; ++ks) put(2 * MB * KS + mb * KL + ks, op->hL, NF, mb, ks);
-  DGC_HIP_TRY(hipMalloc(&op->opF, h.size() * sizeof(float)));
-  DGC_HIP_TRY(hipMemcpy(op->opF, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  const std::vector<float> hf(h.begin(), h.end());
+  DGC_HIP_TRY(hipMalloc(&op->opD, h.size() * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&op->opF, hf.size() * sizeof(float)));
+  DGC_HIP_TRY(hipMemcpy(op->opD, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(op->opF, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
   return DGC_OK;
 }
 
 static void free_op(dgc_op_s* op) {
   if (op->graph) hipGraphExecDestroy(op->graph);
   hipFree(op->opF);
+  hipFree(op->opD);
   hipFree(op->DrT);
   hipFree(op->DsT);
   hipFree(op->LT);
@@ -326,6 +331,7 @@ static int create_op(dgc_disc_t disc, int physics, const double* Dr, const doubl
   op->bc = bc;
   op->wt = dgc::default_waves_per_tile(disc->order, physics);
   int rc = upload_operators(op, Dr, Ds, lift);
+  if (rc == DGC_OK && physics == dgc::kAdvection) rc = build_mfma_operands(op);
   if (rc != DGC_OK) {
     free_op(op);
     return rc;
@@ -372,6 +378,7 @@ static dgc::StageParams base_params(const dgc_op_s* op) {
   p.ay = op->ay;
   p.tclock = nullptr;
   p.opF = op->precision == 1 ? op->opF : nullptr;
+  p.opD = op->opD;
   return p;
 }
 
@@ -568,7 +575,8 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
 
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_waves_per_tile: null operator");
-  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4, "dgc_op_set_waves_per_tile: wt must be 1, 2 or 4");
+  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == dgc::kMfmaTiling,
+              "dgc_op_set_waves_per_tile: wt must be 1, 2, 4 or 0 (FP64 matrix-core variant)");
   op->wt = dgc::supported_waves_per_tile(op->disc->order, op->physics, wt);
   if (op->graph) {
     (void)hipGraphExecDestroy(op->graph);
@@ -584,10 +592,6 @@ int dgc_op_set_precision(dgc_op_t op, int precision) {
   DGC_REQUIRE(precision == 0 || precision == 1, "dgc_op_set_precision: precision must be 0 (FP64) or 1 (FP32 MFMA)");
   DGC_REQUIRE(precision == 0 || op->physics == dgc::kAdvection,
               "dgc_op_set_precision: the FP32 matrix-core form is built for the advection operator");
-  if (precision == 1 && op->opF == nullptr) {
-    int rc = build_mfma_operands(op);
-    if (rc != DGC_OK) return rc;
-  }
   op->precision = precision;
   if (op->graph) {
     (void)hipGraphExecDestroy(op->graph);
@@ -603,7 +607,7 @@ int dgc_op_precision(dgc_op_t op) { return op ? op->precision : -1; }
 // written (stage 2 writes wk0 and wk2, which the next step's stage 1
 // re-initialises), so the state is left untouched.
 int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
-                    float* ms_out /* [3]: wt = 1, 2, 4; -1 if not built */) {
+                    float* ms_out /* [4]: wt = 1, 2, 4, 0; -1 if not built */) {
   int rc = check_step_args(op, inout, size, wk, "dgc_op_autotune");
   if (rc != DGC_OK) return rc;
   DGC_REQUIRE(reps > 0, "dgc_op_autotune: reps must be positive");
@@ -618,13 +622,13 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
   }
   p.c1 = 0.0;
   hipStream_t st = op->disc->ctx->stream;
-  const int cand[3] = {1, 2, 4};
+  const int cand[4] = {1, 2, 4, dgc::kMfmaTiling};
   float best = 0.f;
   int best_wt = op->wt;
   hipEvent_t e0, e1;
   DGC_HIP_TRY(hipEventCreate(&e0));
   DGC_HIP_TRY(hipEventCreate(&e1));
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 4; ++i) {
     const int wt = cand[i];
     if (ms_out) ms_out[i] = -1.f;
     if (dgc::supported_waves_per_tile(op->disc->order, op->physics, wt) != wt) continue;

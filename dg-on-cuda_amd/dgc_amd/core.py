@@ -411,11 +411,12 @@ class _Operator:
         check(lib().dgc_op_set_precision(self._h, {"fp64": 0, "fp32-mfma": 1}[value]))
 
     def autotune(self, state, work, reps=20) -> dict:
-        """Time every built tiling on these buffers and keep the fastest."""
-        ms = (C.c_float * 3)()
+        """Time every built tiling on these buffers and keep the fastest (0 is the
+        advection operator's FP64 matrix-core variant)."""
+        ms = (C.c_float * 4)()
         check(lib().dgc_op_autotune(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
                                     reps, ms))
-        return dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4), ms) if m >= 0})
+        return dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4, 0), ms) if m >= 0})
 
     def profile_stages(self, state, t0, dt, nsteps, work):
         """Mean duration (ms) of each of the 4 stage kernels over nsteps steps."""

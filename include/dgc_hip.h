@@ -118,11 +118,13 @@ int dgc_op_destroy(dgc_op_t op);
 int dgc_op_num_fields(dgc_op_t op);
 
 /* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4; the
- * reference's launch-shape arguments, main.cpp argv[2], play this role).
+ * reference's launch-shape arguments, main.cpp argv[2], play this role), or
+ * wt = 0 for the advection operator's FP64 matrix-core variant (Dr u, Ds u
+ * and LIFT flux as 16x16x4 f64 MFMA products; FP64 throughout).
  * A tiling whose LDS would not fit falls back to 4. dgc_op_autotune times
  * every built tiling on the caller's buffers (work vectors only are written)
- * and keeps the fastest; ms_out[3] receives the mean stage-2 kernel time of
- * wt = 1, 2, 4 (-1 where not built). No reference counterpart. */
+ * and keeps the fastest; ms_out[4] receives the mean stage-2 kernel time of
+ * wt = 1, 2, 4, 0 (-1 where not built). No reference counterpart. */
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt);
 int dgc_op_waves_per_tile(dgc_op_t op);
 /* Arithmetic of the stage contractions (no reference counterpart).
@@ -274,7 +276,8 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
 
 /* Device self-test of the matrix-core operand layouts the kernels rely on:
  * kind 0: C[32x32] = A[32x8] B[8x32] with bf16 inputs (rounded on the device),
- * kind 1: C[16x16] = A[16x4] B[4x16] in fp32. Row-major host arrays; the
+ * kind 1: C[16x16] = A[16x4] B[4x16] in fp32, kind 2: the same in fp64
+ * (inputs widened, result rounded to fp32). Row-major host arrays; the
  * result is scattered with the documented lane layout. No reference counterpart. */
 int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C);
 

@@ -1,5 +1,5 @@
-"""Matrix-core operand layouts used by the MFMA kernels (bf16 32x32x8 and
-fp32 16x16x4) checked on the device against a host product."""
+"""Matrix-core operand layouts used by the MFMA kernels (bf16 32x32x8, fp32
+16x16x4 and fp64 16x16x4) checked on the device against a host product."""
 import ctypes as C
 
 import numpy as np
@@ -16,7 +16,7 @@ def _bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_mfma_layout(ctx, kind):
     L = dgc_amd.lib()
     L.dgc_selftest_mfma.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -30,4 +30,18 @@ def test_mfma_layout(ctx, kind):
     if kind == 0:
         A, B = _bf16_round(A), _bf16_round(B)
     want = A.astype(np.float64) @ B.astype(np.float64)
+    if kind == 2:
+        # raw registers: find each (lane, register) value in the product and
+        # check the documented fp64 layout, D register r of lane l = D[4r + l/16][l%16]
+        raw = Cm.reshape(64, 4)
+        found = {}
+        for lane in range(64):
+            for r in range(4):
+                i, j = np.unravel_index(np.argmin(np.abs(want - raw[lane, r])), want.shape)
+                assert abs(want[i, j] - raw[lane, r]) < 1e-5
+                found[(lane, r)] = (int(i), int(j))
+        print({k: v for k, v in found.items() if k[0] in (0, 1, 16, 17, 33, 63)})
+        for (lane, r), (i, j) in found.items():
+            assert (i, j) == (4 * r + lane // 16, lane % 16), (lane, r, i, j)
+        return
     np.testing.assert_allclose(Cm, want, rtol=1e-5, atol=1e-5)

@@ -136,6 +136,26 @@ def test_periodic_advection_all_orders_vs_oracle(ctx, N):
     assert rel(got, want) < TOL
 
 
+@pytest.mark.parametrize("N", range(1, 8))
+def test_fp64_matrix_core_variant_vs_oracle(ctx, N):
+    """The advection operator's FP64 MFMA variant (tiling 0) meets the FP64 bar."""
+    mesh = dgc_amd.Mesh.structured(24, 24, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO)
+    op.waves_per_tile = 0
+    assert op.waves_per_tile == 0
+    K, Np = disc.K, disc.Np
+    x, y = mesh.dof_coords(N, node_major=False)
+    u0 = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    dt = 0.5 * mesh.min_edge / (np.hypot(1.0, 0.5) * (N + 1) ** 2)
+    state = upload_fields(ctx, u0, 1, K, Np)
+    op.rk4_steps(state, 0.0, dt, 20, op.new_work())
+    want, _ = Problem.from_mesh(mesh, N).rk4(1, u0, 0.0, dt, nsteps=20, ax=1.0, ay=0.5, bc=1)
+    err = rel(download_fields(state, K, Np), want)
+    print(f"N={N}: FP64 MFMA variant rel L2 vs oracle {err:.2e}")
+    assert err < TOL
+
+
 @pytest.mark.parametrize("N", [1, 3, 7])
 def test_maxwell_all_orders_vs_oracle(ctx, N):
     mesh = dgc_amd.Mesh.structured(10, 10, -1, 1, -1, 1, jitter=0.1, seed=6)
@@ -203,13 +223,14 @@ def test_bad_arguments_are_errors(ctx, c1):
 
 @pytest.mark.parametrize("N", [1, 2, 3, 5])
 def test_every_tiling_gives_identical_results(ctx, N):
-    """The stage kernels' tilings (waves per tile) agree to rounding (the
-    compiler may contract multiply-adds differently per instantiation)."""
+    """The stage kernels' tilings (waves per tile, and 0: the advection
+    operator's FP64 matrix-core variant) agree to rounding (the summation
+    order and multiply-add contraction differ per variant)."""
     mesh = dgc_amd.Mesh.structured(20, 13, 0, 1, 0, 1, jitter=0.05, seed=4, periodic=True)
     disc = dgc_amd.Discretization(ctx, mesh, N)
     results = []
     for phys in ("adv", "mx"):
-        for wt in (1, 2, 4):
+        for wt in (1, 2, 4, 0):
             op = (dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO) if phys == "adv"
                   else dgc_amd.MaxwellTM(disc))
             op.waves_per_tile = wt
@@ -231,5 +252,5 @@ def test_autotune_leaves_state_untouched(ctx):
     u0 = np.random.default_rng(0).standard_normal(disc.size)
     st = [dgc_amd.DeviceArray.from_host(ctx, u0)]
     res = op.autotune(st, op.new_work(), reps=3)
-    assert res["choice"] in (1, 2, 4) and res["ms"]
+    assert res["choice"] in (1, 2, 4, 0) and res["ms"] and 0 in res["ms"]
     np.testing.assert_array_equal(st[0].download(), u0)
