@@ -90,6 +90,13 @@ def mfma_stats(r):
                 peak_tflops=MFMA_PEAK_F32_TFLOPS, frac_issued=round(issued / dur / 1e12 / MFMA_PEAK_F32_TFLOPS, 4))
 
 
+# HBM traffic per launch from FETCH_SIZE and WRITE_SIZE counter runs (scaled by the
+# calibration factors x2.000 and x1.000), keyed by (N, K, waves per tile) of the
+# advection stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
+# profiles/dg_stage_kernel_advection_N3_K2097152_hbm.md hold the runs.
+MEASURED_TRAFFIC = {(7, 131072, 4): 196.8e6, (3, 2097152, 2): 938.1e6}
+
+
 def roofline_for(r, F=1):
     """Roofline of the stage-2 kernel (5 state passes) of order r['N']: its mean
     duration over back-to-back launches on the operator's stream (HIP events),
@@ -97,9 +104,12 @@ def roofline_for(r, F=1):
     b = stage_bytes(r["K"], r["Np"], F, 2)
     dur = r["tune"]["ms"][r["tune"]["choice"]] * 1e-3
     achieved = b / dur / 1e9
+    wt = r["tune"]["choice"]
+    traffic = MEASURED_TRAFFIC.get((r["N"], r["K"], wt))
+    name = (f"dg_stage_mfma_kernel<double, N={r['N']}, stage 2>" if wt == 0 else
+            f"dg_stage_kernel<N={r['N']}, advection, stage 2, {wt} waves per tile>")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=f"dg_stage_kernel<N={r['N']}, advection, stage 2>",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                 bytes_per_launch=int(b), avg_launch_us=round(dur * 1e6, 2))
 
 
