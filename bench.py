@@ -278,7 +278,9 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
         wt_by_order[N] = w
     out = dict(instances=n_inst, total_dofs=dofs, steps=steps, K_min=min(Ks), K_max=max(Ks),
                orders={N: orders.count(N) for N in sorted(set(orders))},
-               cells_per_workgroup_by_order={N: 64 * (4 // w) for N, w in sorted(wt_by_order.items())},
+               kernel_by_order={N: ("FP64 MFMA, 64 cells per workgroup" if w == 0 else
+                                    f"VALU, {64 * (4 // w)} cells per workgroup")
+                                for N, w in sorted(wt_by_order.items())},
                work_items=info["num_items"], grid=info["grid"], lds_bytes=info["lds_bytes"],
                ensemble=dict(value=v_e, ms_per_step=ms_e / steps),
                sequential=dict(value=v_s, ms_per_step=ms_s / steps),

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "dg_stage_mfma.hpp"
 #include "dg_stage_tile.hpp"
 
 namespace dgc {
@@ -39,14 +40,32 @@ __global__ __launch_bounds__(kBlock) void ensemble_stage_kernel(const StageParam
   stage_tile<N, kMaxwellTM, STAGE, WT>(p, uniform(item.y), lds);
 }
 
+// The FP64 matrix-core variant (tiling 0): one 64-cell tile per item.
+template <int N, int STAGE>
+__global__ __launch_bounds__(kBlock) void ensemble_mfma_kernel(const StageParams* __restrict__ params,
+                                                               const int2* __restrict__ items) {
+  __shared__ __align__(16) unsigned char lds[mfma_lds_bytes<double, N, kMaxwellTM>()];
+  ConstParams* base = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(params));
+  const int2 item = items[xcd_group(blockIdx.x, gridDim.x)];
+  const StageParams& p = *(const StageParams*)(base + (uniform(item.x) * 4 + STAGE - 1));
+  stage_tile_mfma<double, N, kMaxwellTM, STAGE>(p, uniform(item.y), lds);
+}
+
 using EnsFn = void (*)(const StageParams*, const int2*);
 
 template <int N, int WT>
 EnsFn stage_of(int stage) {
-  return stage == 1 ? ensemble_stage_kernel<N, WT, 1>
-       : stage == 2 ? ensemble_stage_kernel<N, WT, 2>
-       : stage == 3 ? ensemble_stage_kernel<N, WT, 3>
-                    : ensemble_stage_kernel<N, WT, 4>;
+  if constexpr (WT == kMfmaTiling) {
+    return stage == 1 ? ensemble_mfma_kernel<N, 1>
+         : stage == 2 ? ensemble_mfma_kernel<N, 2>
+         : stage == 3 ? ensemble_mfma_kernel<N, 3>
+                      : ensemble_mfma_kernel<N, 4>;
+  } else {
+    return stage == 1 ? ensemble_stage_kernel<N, WT, 1>
+         : stage == 2 ? ensemble_stage_kernel<N, WT, 2>
+         : stage == 3 ? ensemble_stage_kernel<N, WT, 3>
+                      : ensemble_stage_kernel<N, WT, 4>;
+  }
 }
 
 EnsFn ensemble_fn(int order, int wt, int stage) {
@@ -61,6 +80,13 @@ EnsFn ensemble_fn(int order, int wt, int stage) {
     case 5 * 8 + 4: return stage_of<5, 4>(stage);
     case 6 * 8 + 4: return stage_of<6, 4>(stage);
     case 7 * 8 + 4: return stage_of<7, 4>(stage);
+    case 1 * 8 + kMfmaTiling: return stage_of<1, kMfmaTiling>(stage);
+    case 2 * 8 + kMfmaTiling: return stage_of<2, kMfmaTiling>(stage);
+    case 3 * 8 + kMfmaTiling: return stage_of<3, kMfmaTiling>(stage);
+    case 4 * 8 + kMfmaTiling: return stage_of<4, kMfmaTiling>(stage);
+    case 5 * 8 + kMfmaTiling: return stage_of<5, kMfmaTiling>(stage);
+    case 6 * 8 + kMfmaTiling: return stage_of<6, kMfmaTiling>(stage);
+    case 7 * 8 + kMfmaTiling: return stage_of<7, kMfmaTiling>(stage);
     default: return nullptr;
   }
 }

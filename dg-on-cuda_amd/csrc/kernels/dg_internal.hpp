@@ -59,24 +59,24 @@ struct StageParams {
   double* u[3];
   double* acc[3];
   double* k3[3];
-  // Matrix-core operand blocks of Dr, Ds and LIFT (advection only,
-  // dg_rk_mfma.hip): with opF set the stage runs the FP32-contraction form;
-  // the FP64 blocks opD serve the FP64 matrix-core variant (tiling code 0).
+  // Matrix-core operand blocks of Dr, Ds and LIFT (dg_stage_mfma.hpp): with
+  // opF set the stage runs the FP32-contraction form (advection); the FP64
+  // blocks opD serve the FP64 matrix-core variant (tiling code 0).
   const float* __restrict__ opF;
   const double* __restrict__ opD;
 };
 
-// Tiling code of the FP64 matrix-core variant of the advection stage.
+// Tiling code of the FP64 matrix-core variant of the stage kernels.
 constexpr int kMfmaTiling = 0;
 
-// Number of FP32 operand blocks (16 rows x 4 columns, 64 floats each) of the
-// MFMA stage kernel: Dr and Ds as ceil(Np/16) x ceil(Np/4) blocks, LIFT as
+// Number of operand blocks (16 rows x 4 columns, 64 values each) of the
+// matrix-core stage: Dr and Ds as ceil(Np/16) x ceil(Np/4) blocks, LIFT as
 // ceil(Np/16) x ceil(3Nfp/4) blocks.
 inline int mfma_operand_blocks(int Np, int Nfp) {
   const int MB = (Np + 15) / 16, KS = (Np + 3) / 4, KL = (3 * Nfp + 3) / 4;
   return MB * (2 * KS + KL);
 }
-hipError_t launch_stage_mfma(int order, int stage, const StageParams& p, hipStream_t s);
+hipError_t launch_stage_mfma(int order, int physics, int stage, const StageParams& p, hipStream_t s);
 
 // Batched Maxwell TM ensemble (dg_ensemble.hip): one stage of every instance
 // of one order. params: 4 stage parameter records per instance (stages 1..4);

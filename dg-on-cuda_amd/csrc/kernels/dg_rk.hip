@@ -68,7 +68,7 @@ StageFn pick(int order, int wt, int stage) {
 
 // The waves-per-tile value (or kMfmaTiling) that will actually run for a request.
 int supported_waves_per_tile(int order, int physics, int wt) {
-  if (wt == kMfmaTiling) return physics == kAdvection && order >= 1 && order <= kMaxOrder ? kMfmaTiling : 4;
+  if (wt == kMfmaTiling) return order >= 1 && order <= kMaxOrder ? kMfmaTiling : 4;
 #define DGC_WT_CASE(n) \
   case n: return physics == kMaxwellTM ? effective_wt<n, kMaxwellTM>(wt) : effective_wt<n, kAdvection>(wt);
   switch (order) {
@@ -90,12 +90,10 @@ int default_waves_per_tile(int order, int physics) {
 }
 
 hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s) {
-  if (p.opF != nullptr) return physics == kAdvection ? launch_stage_mfma(order, stage, p, s) : hipErrorInvalidValue;
+  if (p.opF != nullptr) return physics == kAdvection ? launch_stage_mfma(order, physics, stage, p, s) : hipErrorInvalidValue;
   if (wt == kMfmaTiling) {
-    if (physics != kAdvection || p.opD == nullptr) return hipErrorInvalidValue;
-    StageParams q = p;
-    q.opF = nullptr;
-    return launch_stage_mfma(order, stage, q, s);
+    if (p.opD == nullptr) return hipErrorInvalidValue;
+    return launch_stage_mfma(order, physics, stage, p, s);
   }
   if (stage < 0 || stage > 4 || (wt != 1 && wt != 2 && wt != 4)) return hipErrorInvalidValue;
   wt = supported_waves_per_tile(order, physics, wt);

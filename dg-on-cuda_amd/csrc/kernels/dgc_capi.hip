@@ -331,7 +331,7 @@ static int create_op(dgc_disc_t disc, int physics, const double* Dr, const doubl
   op->bc = bc;
   op->wt = dgc::default_waves_per_tile(disc->order, physics);
   int rc = upload_operators(op, Dr, Ds, lift);
-  if (rc == DGC_OK && physics == dgc::kAdvection) rc = build_mfma_operands(op);
+  if (rc == DGC_OK) rc = build_mfma_operands(op);
   if (rc != DGC_OK) {
     free_op(op);
     return rc;
@@ -792,7 +792,7 @@ static int ensemble_build(dgc_ensemble_s* e) {
     }
     EnsembleGroup& g = e->groups.back();
     e->wt[i] = g.wt;
-    const int cells = dgc::kTile * (dgc::kWaves / g.wt);
+    const int cells = g.wt == dgc::kMfmaTiling ? dgc::kTile : dgc::kTile * (dgc::kWaves / g.wt);
     const int ngroups = (e->ops[i]->disc->K + cells - 1) / cells;
     for (int t = 0; t < ngroups; ++t) items.push_back(make_int2(i, t));
     g.nitems = int(items.size()) - g.offset;
@@ -878,7 +878,8 @@ int dgc_ensemble_info(dgc_ensemble_t e, int* wt, int* num_items, int* grid, size
   size_t lds = 0;
   for (const auto& gr : e->groups) {
     g += gr.nitems;
-    lds = std::max(lds, size_t(dgc::stage_lds_doubles(gr.order, dgc::kMaxwellTM, gr.wt)) * sizeof(double));
+    if (gr.wt != dgc::kMfmaTiling)
+      lds = std::max(lds, size_t(dgc::stage_lds_doubles(gr.order, dgc::kMaxwellTM, gr.wt)) * sizeof(double));
   }
   if (grid) *grid = g;
   if (lds_bytes) *lds_bytes = lds;

@@ -119,8 +119,8 @@ int dgc_op_num_fields(dgc_op_t op);
 
 /* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4; the
  * reference's launch-shape arguments, main.cpp argv[2], play this role), or
- * wt = 0 for the advection operator's FP64 matrix-core variant (Dr u, Ds u
- * and LIFT flux as 16x16x4 f64 MFMA products; FP64 throughout).
+ * wt = 0 for the FP64 matrix-core variant (Dr u, Ds u and LIFT flux as
+ * 16x16x4 f64 MFMA products, one 64-cell tile per workgroup; FP64 throughout).
  * A tiling whose LDS would not fit falls back to 4. dgc_op_autotune times
  * every built tiling on the caller's buffers (work vectors only are written)
  * and keeps the fastest; ms_out[4] receives the mean stage-2 kernel time of

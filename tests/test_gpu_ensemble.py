@@ -57,6 +57,25 @@ def test_ensemble_equals_per_instance_kernels_and_oracle(ctx):
             assert np.linalg.norm(got - oracle) / np.linalg.norm(oracle) < 1e-12
 
 
+def test_matrix_core_tiling_in_the_ensemble(ctx):
+    """Instances on the FP64 matrix-core variant (tiling 0) give what their own
+    per-order matrix-core kernels give."""
+    inst = _instances(ctx, seed=2)
+    for _, _, op, _, _ in inst:
+        op.waves_per_tile = 0
+    states = [_upload(ctx, u0, d.K, d.Np) for (_, d, _, u0, _) in inst]
+    works = [op.new_work() for (_, _, op, _, _) in inst]
+    ens = dgc_amd.Ensemble(ctx, [i[2] for i in inst], states, works, [i[4] for i in inst])
+    assert ens.info()["waves_per_tile"] == [0] * len(inst)
+    ens.steps(5)
+    for k, (mesh, disc, op, u0, dt) in enumerate(inst):
+        ref_state = _upload(ctx, u0, disc.K, disc.Np)
+        op.rk4_steps(ref_state, 0.0, dt, 5, op.new_work())
+        got = _download(states[k], disc.K, disc.Np)
+        want = _download(ref_state, disc.K, disc.Np)
+        assert np.max(np.abs(got - want)) == 0.0, k
+
+
 def test_autotuned_and_sequential_modes_agree(ctx):
     inst = _instances(ctx, seed=1)
     states = [_upload(ctx, u0, d.K, d.Np) for (_, d, _, u0, _) in inst]

@@ -156,6 +156,40 @@ def test_fp64_matrix_core_variant_vs_oracle(ctx, N):
     assert err < TOL
 
 
+@pytest.mark.parametrize("N", [1, 2, 4, 5, 6, 7])
+def test_maxwell_fp64_matrix_core_variant_vs_oracle(ctx, N):
+    mesh = dgc_amd.Mesh.structured(10, 10, -1, 1, -1, 1, jitter=0.1, seed=6)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.MaxwellTM(disc)
+    op.waves_per_tile = 0
+    assert op.waves_per_tile == 0
+    K, Np = disc.K, disc.Np
+    u0 = np.random.default_rng(N).standard_normal(3 * K * Np)
+    dt = 0.5 * mesh.min_edge / (N + 1) ** 2
+    state = upload_fields(ctx, u0, 3, K, Np)
+    op.rk4_steps(state, 0.0, dt, 10, op.new_work())
+    want, _ = Problem.from_mesh(mesh, N).rk4(0, u0, 0.0, dt, nsteps=10)
+    err = rel(download_fields(state, K, Np), want)
+    print(f"N={N}: Maxwell FP64 MFMA variant rel L2 vs oracle {err:.2e}")
+    assert err < TOL
+
+
+def test_maxwell_cavity_t1_fp64_matrix_core_variant(ctx, c1):
+    fix, mesh, disc, op = c1
+    K, Np = disc.K, disc.Np
+    n = K * Np
+    u0 = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    op.waves_per_tile = 0
+    try:
+        state = upload_fields(ctx, u0, 3, K, Np)
+        t, steps = op.rk4_run(state, 0.0, 1.0, float(fix["dt"]), op.new_work())
+        err = rel(download_fields(state, K, Np), fix["final"])
+        print(f"maxwell C1 t=1 (FP64 MFMA variant) rel L2 vs reference = {err:.3e}")
+        assert steps == int(fix["steps"]) and err < TOL
+    finally:
+        op.waves_per_tile = 4
+
+
 @pytest.mark.parametrize("N", [1, 3, 7])
 def test_maxwell_all_orders_vs_oracle(ctx, N):
     mesh = dgc_amd.Mesh.structured(10, 10, -1, 1, -1, 1, jitter=0.1, seed=6)
@@ -223,9 +257,9 @@ def test_bad_arguments_are_errors(ctx, c1):
 
 @pytest.mark.parametrize("N", [1, 2, 3, 5])
 def test_every_tiling_gives_identical_results(ctx, N):
-    """The stage kernels' tilings (waves per tile, and 0: the advection
-    operator's FP64 matrix-core variant) agree to rounding (the summation
-    order and multiply-add contraction differ per variant)."""
+    """The stage kernels' tilings (waves per tile, and 0: the FP64
+    matrix-core variant) agree to rounding (the summation order and
+    multiply-add contraction differ per variant)."""
     mesh = dgc_amd.Mesh.structured(20, 13, 0, 1, 0, 1, jitter=0.05, seed=4, periodic=True)
     disc = dgc_amd.Discretization(ctx, mesh, N)
     results = []
