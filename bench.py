@@ -154,6 +154,110 @@ def cpu_baseline(dgc, N=3, steps=40):
                        f"in {secs:.2f} s on 1 host core")
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_all_cores(dgc, N=3, steps=100):
+    """The restatement of the reference advection path with its cell loops on
+    all host cores (OpenMP), same lattice and order as the serial baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Problem
+
+    mesh = dgc.Mesh.structured(256, 256, -1, 1, -1, 1, jitter=0.05, seed=1)
+    Np = (N + 1) * (N + 2) // 2
+    dt = 0.5 * mesh.min_edge / (np.sqrt(2.0) * (N + 1) ** 2)
+    P = Problem.from_mesh(mesh, N, threaded=True)
+    x, y = mesh.dof_coords(N, node_major=False)
+    u0 = np.sin((x + 1) * np.pi) * np.sin((y + 1) * np.pi)
+    P.rk4(1, u0, 0.0, dt, nsteps=2, ax=1.0, ay=1.0, bc=0)
+    t0 = time.perf_counter()
+    P.rk4(1, u0, 0.0, dt, nsteps=steps, ax=1.0, ay=1.0, bc=0)
+    secs = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=mesh.K * Np * 4 * steps / secs, unit="DOF-stage updates/s", cores=cores, kind="port",
+                cpu=cpu_model(), nproc=os.cpu_count(),
+                sample=f"advection_2d restatement (a=(1,1), inflow BC), K={mesh.K}, N={N}, {steps} RK4 steps in "
+                       f"{secs:.2f} s, cell loops on {cores} OpenMP threads")
+
+
+def stream_probe(dgc, ctx, reps=20):
+    """Achievable HBM rate on this box: out = a x + y over 2^27 doubles per
+    array (reads 2 GiB, writes 1 GiB; 3 x 1 GiB vectors exceed the 256 MB cache)."""
+    n = 1 << 27
+    x, y, o = (dgc.DeviceArray(ctx, n) for _ in range(3))
+    for a in (x, y, o):
+        a.zero()
+    call = lambda: dgc.core.check(dgc.lib().dgc_axpy(ctx.handle, 0.5, x.ptr, n, y.ptr, o.ptr))
+    for _ in range(3):
+        call()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    gbs = 3 * n * 8 * reps / secs / 1e9
+    del x, y, o
+    return dict(kernel="axpy2_kernel", bytes_per_launch=3 * n * 8, GBs=round(gbs, 1),
+                frac_of_8TBs=round(gbs / HBM_PEAK_GBS, 4))
+
+
+def maxwell_c1(dgc, ctx, with_cpu=True):
+    """Maxwell TM cavity: 16x16 squares on [-1,1]^2 split into K = 512 triangles,
+    jitter 0.1h (seed 0), N = 4, dt = 0.5 h_min / (N+1)^2, run to t = 1 with a
+    truncated last step, Ez = sin(pi x) sin(pi y); GPU time and, beside it, the
+    reference CPU path on the same mesh and steps."""
+    N = 4
+    mesh = dgc.Mesh.structured(16, 16, -1, 1, -1, 1, jitter=0.1, seed=0)
+    disc = dgc.Discretization(ctx, mesh, N)
+    op = dgc.MaxwellTM(disc)
+    x, y = mesh.dof_coords(N, node_major=True)
+    ez = np.sin(np.pi * x) * np.sin(np.pi * y)
+    dt = 0.5 * mesh.min_edge / (N + 1) ** 2
+    st = [dgc.DeviceArray(ctx, disc.size), dgc.DeviceArray(ctx, disc.size), dgc.DeviceArray.from_host(ctx, ez)]
+    st[0].zero()
+    st[1].zero()
+    work = op.new_work()
+    op.rk4_run(st, 0.0, 1.0, dt, work)  # warm-up (graph capture)
+    for a, v in zip(st, (None, None, ez)):
+        a.zero() if v is None else a.upload(v)
+    ctx.sync()
+    t0 = time.perf_counter()
+    t, steps = op.rk4_run(st, 0.0, 1.0, dt, work)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    s2 = np.sqrt(2.0)
+    exact = np.sin(np.pi * x) * np.sin(np.pi * y) * np.cos(s2 * np.pi)
+    err = float(np.mean((st[2].download() - exact) ** 2))
+    dofs = 3 * disc.K * disc.Np * 4 * steps
+    out = dict(K=disc.K, N=N, steps=steps, t=t, gpu_ms=round(secs * 1e3, 3), gpu_value=dofs / secs,
+               ez_mean_square_error_vs_cavity_mode=err, unit="DOF-stage updates/s (3 fields)")
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if with_cpu and os.path.exists(exe):
+        vx, vy, etov = mesh.vertices()
+        with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+            np.array([len(vx), etov.size // 3], np.int32).tofile(f)
+            vx.tofile(f)
+            vy.tofile(f)
+            etov.astype(np.int32).tofile(f)
+            path = f.name
+        try:
+            res = subprocess.run([exe, "maxwell_time", path, str(N), str(steps), repr(dt)],
+                                 capture_output=True, text=True, check=True, timeout=300)
+        finally:
+            os.unlink(path)
+        csecs = float(res.stdout.split()[1])
+        out["cpu_reference"] = dict(value=dofs / csecs, seconds=round(csecs, 3), cores=1, kind="reference")
+    return out
+
+
 def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
     """IPDG Poisson -Lap u = f on [-1,1]^2, u = 0 on the boundary, tau = (N+1)(N+2)/h,
     n_side x n_side squares split into triangles, jitter 0.1h (seed 3). Time to
@@ -386,10 +490,19 @@ def main():
     if not args.no_ensemble:
         extra["ensemble"] = ensemble_bench(dgc, ctx, steps=args.ensemble_steps)
 
+    extra["hbm_stream_probe"] = stream_probe(dgc, ctx)
+    print(f"# STREAM-like probe: {extra['hbm_stream_probe']['GBs']} GB/s", file=sys.stderr, flush=True)
+    extra["maxwell_c1"] = maxwell_c1(dgc, ctx, with_cpu=not args.no_cpu)
+    print(f"# Maxwell C1: {extra['maxwell_c1']['gpu_ms']} ms on the GPU for {extra['maxwell_c1']['steps']} steps",
+          file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dgc)
         print(f"# CPU baseline: {cpu['value'] / 1e6:.2f} M DOF-stage/s ({cpu['kind']})", file=sys.stderr)
+        extra["cpu_baseline_all_cores"] = cpu_baseline_all_cores(dgc)
+        print(f"# CPU baseline, all cores: {extra['cpu_baseline_all_cores']['value'] / 1e6:.2f} M DOF-stage/s "
+              f"({extra['cpu_baseline_all_cores']['cores']} threads)", file=sys.stderr, flush=True)
 
     line = {
         "metric": "DOF-stage updates/s",

@@ -9,8 +9,11 @@ here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 ref="${DGC_REFERENCE:-/root/reference}"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 
-g++ -O3 -std=c++17 -fPIC -shared -Wall -o "$here/liboracle.so.tmp" "$here/dg_oracle.cpp"
+g++ -O3 -std=c++17 -fPIC -shared -Wall -Wno-unknown-pragmas -o "$here/liboracle.so.tmp" "$here/dg_oracle.cpp"
 mv "$here/liboracle.so.tmp" "$here/liboracle.so"
+# the same restatement with its cell loops threaded (all-core CPU baseline)
+g++ -O3 -std=c++17 -fPIC -shared -Wall -fopenmp -o "$here/liboracle_omp.so.tmp" "$here/dg_oracle.cpp"
+mv "$here/liboracle_omp.so.tmp" "$here/liboracle_omp.so"
 
 if [[ -d "$ref/src" ]]; then
   mkdir -p "$here/_ref"

@@ -14,6 +14,11 @@
 // Pinning: tests/test_oracle.py checks every function here against the
 // golden vectors written by the compiled reference (tests/golden/), and
 // against closed forms (polynomial exactness, energy, symmetry).
+//
+// The cell loops carry OpenMP pragmas (per-cell work, no reductions, so a
+// threaded build gives the same bits); liboracle.so is built without
+// -fopenmp (serial, as the reference), liboracle_omp.so with it, for the
+// all-core CPU baseline.
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -146,6 +151,7 @@ void oracle_maxwell_rhs(int K, int N, const double* Dr, const double* Ds, const 
   double *outHx = out, *outHy = out + n, *outEz = out + 2 * n;
   std::vector<double> nfHx(size_t(K) * 3 * Nfp), nfHy(nfHx.size()), nfEz(nfHx.size());
   // numerical fluxes (maxwell_2d.h:179-256)
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < K; ++c) {
     const Cell cell = get_cell(cx, cy, c);
     for (int e = 0; e < 3; ++e) {
@@ -175,7 +181,10 @@ void oracle_maxwell_rhs(int K, int N, const double* Dr, const double* Ds, const 
       }
     }
   }
+#pragma omp parallel
+  {
   std::vector<double> Dx(Np * Np), Dy(Np * Np), fHx(3 * Nfp), fHy(3 * Nfp), fEz(3 * Nfp);
+#pragma omp for schedule(static)
   for (int c = 0; c < K; ++c) {
     const Cell cell = get_cell(cx, cy, c);
     double ij[4];
@@ -201,6 +210,7 @@ void oracle_maxwell_rhs(int K, int N, const double* Dr, const double* Ds, const 
     gemv(L, Np, 3 * Nfp, -1.0 / J, fHy.data(), 1.0, outHy + o);
     gemv(L, Np, 3 * Nfp, -1.0 / J, fEz.data(), 1.0, outEz + o);
   }
+  }
 }
 
 // Advection RHS: examples/advection_2d/advection_2d.h:156-258, generalised
@@ -213,6 +223,7 @@ void oracle_advection_rhs(int K, int N, const double* Dr, const double* Ds, cons
                           double ay, int bc, const double* in, double t, double* out) {
   const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
   std::vector<double> nf(size_t(K) * 3 * Nfp);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < K; ++c) {
     const Cell cell = get_cell(cx, cy, c);
     for (int e = 0; e < 3; ++e) {
@@ -241,7 +252,10 @@ void oracle_advection_rhs(int K, int N, const double* Dr, const double* Ds, cons
       }
     }
   }
+#pragma omp parallel
+  {
   std::vector<double> Dx(Np * Np), Dy(Np * Np), mFl(3 * Nfp);
+#pragma omp for schedule(static)
   for (int c = 0; c < K; ++c) {
     const Cell cell = get_cell(cx, cy, c);
     double ij[4];
@@ -256,6 +270,7 @@ void oracle_advection_rhs(int K, int N, const double* Dr, const double* Ds, cons
       for (int i = 0; i < Nfp; ++i) mFl[e * Nfp + i] = nf[size_t(c) * 3 * Nfp + e * Nfp + i] * fj;
     }
     gemv(L, Np, 3 * Nfp, -1.0 / cell_J(cell), mFl.data(), 1.0, out + o);
+  }
   }
 }
 
@@ -371,6 +386,7 @@ int oracle_rk4(int phys, int K, int N, const double* Dr, const double* Ds, const
       oracle_advection_rhs(K, N, Dr, Ds, L, cx, cy, nb_cell, nb_face, r, s, ax, ay, bc, in, t, out);
   };
   auto axpy = [&](double a, const double* x, const double* y, double* o) {
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; ++i) o[i] = a * x[i] + y[i];
   };
   auto step = [&](double t, double h) {

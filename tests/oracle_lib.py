@@ -12,17 +12,18 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _PATH = os.path.join(ROOT, "oracle", "liboracle.so")
-_L = None
+_PATH_OMP = os.path.join(ROOT, "oracle", "liboracle_omp.so")  # threaded cell loops
+_LIBS = {}
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
-def lib():
-    global _L
-    if _L is None:
-        if not os.path.exists(_PATH):
+def lib(threaded=False):
+    path = _PATH_OMP if threaded else _PATH
+    if path not in _LIBS:
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: run oracle/build_ref.sh (or __graft_entry__.build())")
-        L = C.CDLL(_PATH)
+        L = C.CDLL(path)
         L.oracle_maxwell_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
                                          C.c_double, _dp]
         L.oracle_maxwell_rhs.restype = None
@@ -36,8 +37,8 @@ def lib():
                                  C.c_double, C.c_double, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
                                  C.c_double]
         L.oracle_rk4.restype = C.c_int
-        _L = L
-    return _L
+        _LIBS[path] = L
+    return _LIBS[path]
 
 
 def _d(a):
@@ -53,7 +54,8 @@ def _i(a):
 class Problem:
     """Mesh + operators in the oracle's (reference CPU) conventions."""
 
-    def __init__(self, order, Dr, Ds, L, cx, cy, nb_cell, nb_face, r=None, s=None):
+    def __init__(self, order, Dr, Ds, L, cx, cy, nb_cell, nb_face, r=None, s=None, threaded=False):
+        self._lib = lib(threaded)
         self.N = order
         self.Np = (order + 1) * (order + 2) // 2
         self.K = len(cx) // 3
@@ -65,38 +67,38 @@ class Problem:
         self.s = c(s if s is not None else np.zeros(self.Np), np.float64)
 
     @classmethod
-    def from_mesh(cls, mesh, order, form="weak", ops=None):
+    def from_mesh(cls, mesh, order, form="weak", ops=None, threaded=False):
         import dgc_amd
 
         ops = ops if ops is not None else dgc_amd.ref_operators(order, form)
         cells = mesh.cells()
         return cls(order, ops["Dr"], ops["Ds"], ops["LIFT"], cells["cx"], cells["cy"], cells["nb_cell"],
-                   cells["nb_face"], ops["r"], ops["s"])
+                   cells["nb_face"], ops["r"], ops["s"], threaded=threaded)
 
     def maxwell_rhs(self, u, t=0.0):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros_like(u)
-        lib().oracle_maxwell_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+        self._lib.oracle_maxwell_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                  _d(self.cy), _i(self.nb_cell), _i(self.nb_face), _d(u), t, _d(out))
         return out
 
     def advection_rhs(self, u, t=0.0, ax=1.0, ay=1.0, bc=0):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros_like(u)
-        lib().oracle_advection_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+        self._lib.oracle_advection_rhs(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                    _d(self.cy), _i(self.nb_cell), _i(self.nb_face), _d(self.r),
                                    _d(self.s), ax, ay, bc, _d(u), t, _d(out))
         return out
 
     def laplace(self, u, tau_mode=0):
         v = np.array(u, dtype=np.float64, copy=True)
-        lib().oracle_laplace_apply(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+        self._lib.oracle_laplace_apply(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                    _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, _d(v))
         return v
 
     def rk4(self, phys, u, t0, dt, nsteps=0, t_end=-1.0, ax=1.0, ay=1.0, bc=0):
         v = np.array(u, dtype=np.float64, copy=True)
-        steps = lib().oracle_rk4(phys, self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L),
+        steps = self._lib.oracle_rk4(phys, self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L),
                                  _d(self.cx), _d(self.cy), _i(self.nb_cell), _i(self.nb_face),
                                  _d(self.r), _d(self.s), ax, ay, bc, _d(v), t0, dt, nsteps, t_end)
         return v, steps

@@ -288,3 +288,22 @@ def test_autotune_leaves_state_untouched(ctx):
     res = op.autotune(st, op.new_work(), reps=3)
     assert res["choice"] in (1, 2, 4, 0) and res["ms"] and 0 in res["ms"]
     np.testing.assert_array_equal(st[0].download(), u0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, (1 << 20) + 3])
+def test_axpy_matches_host(ctx, n):
+    """dgc_axpy (out = a x + y, the k_axpy replacement) on even, odd and
+    16-byte-misaligned vectors: within one rounding of the exact result."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n + 1)
+    y = rng.standard_normal(n + 1)
+    a = 0.37
+    dx, dy, do = (dgc_amd.DeviceArray.from_host(ctx, v) for v in (x, y, np.zeros(n + 1)))
+    L = dgc_amd.lib()
+    for off in (0, 1):  # off = 1: pointers 8 bytes past a 16-byte boundary
+        rc = L.dgc_axpy(ctx.handle, a, dx.ptr + 8 * off, n, dy.ptr + 8 * off, do.ptr + 8 * off)
+        assert rc == 0
+        got = do.download()[off:off + n]
+        want = a * x[off:off + n] + y[off:off + n]
+        bound = 2.3e-16 * (np.abs(a * x[off:off + n]) + np.abs(y[off:off + n]))
+        assert np.all(np.abs(got - want) <= bound)

@@ -144,3 +144,19 @@ def test_maxwell_energy_nonincreasing_and_advection_conservation():
     assert sum(vv[c] @ Mp[c] @ dv[c] for c in range(Pp.K)) < 0
     const = Pp.advection_rhs(np.ones(Pp.K * Pp.Np), 0.0, 1.0, 0.5, 1)
     assert np.abs(const).max() < 1e-10
+
+
+def test_threaded_oracle_gives_the_same_bits():
+    """The all-core build of the restatement (OpenMP cell loops) changes no bit."""
+    import dgc_amd
+    from oracle_lib import Problem
+
+    mesh = dgc_amd.Mesh.structured(12, 10, -1, 1, -1, 1, jitter=0.1, seed=4)
+    for phys, N in ((0, 3), (1, 5)):
+        P1 = Problem.from_mesh(mesh, N)
+        P2 = Problem.from_mesh(mesh, N, threaded=True)
+        F = 3 if phys == 0 else 1
+        u0 = np.random.default_rng(N).standard_normal(F * P1.K * P1.Np)
+        a, _ = P1.rk4(phys, u0, 0.0, 1e-3, nsteps=3, ax=1.0, ay=0.5, bc=1)
+        b, _ = P2.rk4(phys, u0, 0.0, 1e-3, nsteps=3, ax=1.0, ay=0.5, bc=1)
+        np.testing.assert_array_equal(a, b)
