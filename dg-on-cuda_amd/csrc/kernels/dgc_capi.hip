@@ -79,9 +79,9 @@ int dgc_ctx_create(int device, dgc_ctx_t* out) {
 
 int dgc_ctx_destroy(dgc_ctx_t ctx) {
   if (ctx == nullptr) return DGC_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DGC_OK;
 }
@@ -198,12 +198,12 @@ int dgc_disc2d_create(dgc_ctx_t ctx, int num_cells, int order, const int* face0_
   d->h_face_J.assign(face_J, face_J + 3 * size_t(K));
   d->h_nbr = nbr;
   auto fail = [&](hipError_t e, const char* what) {
-    hipFree(d->geo);
-    hipFree(d->nbr);
-    hipFree(d->J);
-    hipFree(d->face_J);
-    hipFree(d->bx);
-    hipFree(d->by);
+    (void)hipFree(d->geo);
+    (void)hipFree(d->nbr);
+    (void)hipFree(d->J);
+    (void)hipFree(d->face_J);
+    (void)hipFree(d->bx);
+    (void)hipFree(d->by);
     delete d;
     return dgc_set_error(DGC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
@@ -237,14 +237,14 @@ int dgc_disc2d_create(dgc_ctx_t ctx, int num_cells, int order, const int* face0_
 
 int dgc_disc2d_destroy(dgc_disc_t disc) {
   if (disc == nullptr) return DGC_OK;
-  hipSetDevice(disc->ctx->device);
-  hipStreamSynchronize(disc->ctx->stream);
-  hipFree(disc->geo);
-  hipFree(disc->nbr);
-  hipFree(disc->J);
-  hipFree(disc->face_J);
-  hipFree(disc->bx);
-  hipFree(disc->by);
+  (void)hipSetDevice(disc->ctx->device);
+  (void)hipStreamSynchronize(disc->ctx->stream);
+  (void)hipFree(disc->geo);
+  (void)hipFree(disc->nbr);
+  (void)hipFree(disc->J);
+  (void)hipFree(disc->face_J);
+  (void)hipFree(disc->bx);
+  (void)hipFree(disc->by);
   delete disc;
   return DGC_OK;
 }
@@ -305,13 +305,13 @@ static int build_mfma_operands(dgc_op_s* op) {
 }
 
 static void free_op(dgc_op_s* op) {
-  if (op->graph) hipGraphExecDestroy(op->graph);
-  hipFree(op->opF);
-  hipFree(op->opD);
-  hipFree(op->DrT);
-  hipFree(op->DsT);
-  hipFree(op->LT);
-  hipFree(op->tclock);
+  if (op->graph) (void)hipGraphExecDestroy(op->graph);
+  (void)hipFree(op->opF);
+  (void)hipFree(op->opD);
+  (void)hipFree(op->DrT);
+  (void)hipFree(op->DsT);
+  (void)hipFree(op->LT);
+  (void)hipFree(op->tclock);
   delete op;
 }
 
@@ -354,8 +354,8 @@ int dgc_advection_create(dgc_disc_t disc, const double* Dr, const double* Ds, co
 
 int dgc_op_destroy(dgc_op_t op) {
   if (op == nullptr) return DGC_OK;
-  hipSetDevice(op->disc->ctx->device);
-  hipStreamSynchronize(op->disc->ctx->stream);
+  (void)hipSetDevice(op->disc->ctx->device);
+  (void)hipStreamSynchronize(op->disc->ctx->stream);
   free_op(op);
   return DGC_OK;
 }
@@ -528,7 +528,7 @@ static int step_graph(dgc_op_s* op, double* const* u, double dt, double* const* 
   for (int i = 0; i < 4 * op->F; ++i) key.push_back(wk[i]);
   if (op->graph && key == op->graph_key && op->graph_dt == dt) return DGC_OK;
   if (op->graph) {
-    hipGraphExecDestroy(op->graph);
+    (void)hipGraphExecDestroy(op->graph);
     op->graph = nullptr;
   }
   hipStream_t s = op->disc->ctx->stream;
@@ -540,12 +540,12 @@ static int step_graph(dgc_op_s* op, double* const* u, double dt, double* const* 
   }
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != DGC_OK) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&op->graph, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
     op->graph = nullptr;
     return dgc_set_error(DGC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));

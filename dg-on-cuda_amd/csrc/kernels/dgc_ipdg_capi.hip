@@ -290,7 +290,7 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
     L.graph_coarse = L.use_coarse;
   }
   T h[dgc::kCgScalars];
-  for (int done_chunks = 0;; ++done_chunks) {
+  for (;;) {
     DGC_HIP_TRY(hipMemcpyAsync(h, L.scal, sizeof(h), hipMemcpyDeviceToHost, st));
     DGC_HIP_TRY(hipStreamSynchronize(st));
     if (h[dgc::kDone] != T(0)) break;
