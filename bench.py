@@ -9,8 +9,9 @@ N = 1..7. One bench "step" is one RK4 step of every order in the sweep; the
 sweep's whole-job rate is  sum_N (K Np 4 steps) / sum_N (wall time).
 
 Alongside: the roofline of the dominant kernel (HIP events around each stage
-kernel; algorithmic bytes per launch), an HBM-resident variant (K = 2.1M at
-N = 3, larger than the 256 MB Infinity Cache), the reference CPU path
+kernel; algorithmic bytes per launch), HBM-resident low-order runs (advection
+and Maxwell TM at N = 1..3 on 2048 x 2048 squares, K = 8.4M: working sets of
+0.8-6 GB, far above the 256 MB Infinity Cache), the reference CPU path
 timed on a bounded sample on the host cores, and the IPDG Poisson solvers'
 time to FP64 relative residual 1e-10 (block-Jacobi PCG, two-level PCG and
 flexible CG with the learned preconditioner) at K = 32768 N = 3 and
@@ -53,9 +54,24 @@ def setup_advection(dgc, ctx, mesh, N, a=(1.0, 0.5)):
     return disc, op, state, work, dt, u0
 
 
-def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64"):
-    disc, op, state, work, dt, u0 = setup_advection(dgc, ctx, mesh, N)
-    K, Np = disc.K, disc.Np
+def setup_maxwell(dgc, ctx, mesh, N):
+    """Maxwell TM, Ez = sin(pi x) sin(pi y), Hx = Hy = 0, dt = 0.5 h_min / (N+1)^2."""
+    disc = dgc.Discretization(ctx, mesh, N)
+    op = dgc.MaxwellTM(disc)
+    x, y = mesh.dof_coords(N, node_major=True)
+    ez = np.sin(np.pi * x) * np.sin(np.pi * y)
+    state = [dgc.DeviceArray(ctx, disc.size), dgc.DeviceArray(ctx, disc.size), dgc.DeviceArray.from_host(ctx, ez)]
+    state[0].zero()
+    state[1].zero()
+    work = op.new_work()
+    dt = 0.5 * mesh.min_edge / (N + 1) ** 2
+    return disc, op, state, work, dt, ez
+
+
+def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64", physics="advection"):
+    setup = setup_maxwell if physics == "maxwell" else setup_advection
+    disc, op, state, work, dt, u0 = setup(dgc, ctx, mesh, N)
+    K, Np, F = disc.K, disc.Np, op.F
     op.precision = precision
     tune = op.autotune(state, work)  # picks the tiling (waves per 64-cell tile)
     if warmup:
@@ -66,11 +82,11 @@ def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64"):
     ctx.sync()
     wall = time.perf_counter() - t0
     stage_ms = op.profile_stages(state, 0.0, dt, prof_steps, work)
-    u = state[0].download()
+    u = state[-1].download()
     ok = bool(np.all(np.isfinite(u)))
-    dofs = K * Np * 4 * steps
-    return dict(N=N, Np=Np, K=K, dt=dt, wall_s=wall, event_ms=ev_ms, dof_stage=dofs,
-                value=dofs / wall, stage_ms=stage_ms, finite=ok, tune=tune, u=u)
+    dofs = F * K * Np * 4 * steps
+    return dict(N=N, Np=Np, K=K, F=F, physics=physics, dt=dt, wall_s=wall, event_ms=ev_ms, dof_stage=dofs,
+                value=dofs / wall, stage_ms=stage_ms, finite=ok, tune=tune, streaming=op.streaming, u=u)
 
 
 MFMA_PEAK_F32_TFLOPS = 157.3  # MI355X dense FP32 matrix, published
@@ -91,23 +107,25 @@ def mfma_stats(r):
 
 
 # HBM traffic per launch from FETCH_SIZE and WRITE_SIZE counter runs (scaled by the
-# calibration factors x2.000 and x1.000), keyed by (N, K, waves per tile) of the
-# advection stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
+# calibration factors x2.000 and x1.000), keyed by (physics, N, K, waves per tile)
+# of the stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
 # profiles/dg_stage_kernel_advection_N3_K2097152_hbm.md hold the runs.
-MEASURED_TRAFFIC = {(7, 131072, 4): 196.8e6, (3, 2097152, 2): 938.1e6}
+MEASURED_TRAFFIC = {("advection", 7, 131072, 4): 196.8e6, ("advection", 3, 2097152, 2): 938.1e6}
 
 
-def roofline_for(r, F=1):
+def roofline_for(r):
     """Roofline of the stage-2 kernel (5 state passes) of order r['N']: its mean
     duration over back-to-back launches on the operator's stream (HIP events),
     the figure rocprof's kernel trace reports for it."""
-    b = stage_bytes(r["K"], r["Np"], F, 2)
+    b = stage_bytes(r["K"], r["Np"], r.get("F", 1), 2)
     dur = r["tune"]["ms"][r["tune"]["choice"]] * 1e-3
     achieved = b / dur / 1e9
     wt = r["tune"]["choice"]
-    traffic = MEASURED_TRAFFIC.get((r["N"], r["K"], wt))
-    name = (f"dg_stage_mfma_kernel<double, N={r['N']}, stage 2>" if wt == 0 else
-            f"dg_stage_kernel<N={r['N']}, advection, stage 2, {wt} waves per tile>")
+    phys = r.get("physics", "advection")
+    traffic = MEASURED_TRAFFIC.get((phys, r["N"], r["K"], wt))
+    hint = ", nontemporal streams" if r.get("streaming") else ""
+    name = (f"dg_stage_mfma_kernel<double, N={r['N']}, {phys}, stage 2{hint}>" if wt == 0 else
+            f"dg_stage_kernel<N={r['N']}, {phys}, stage 2, {wt} waves per tile{hint}>")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                 bytes_per_launch=int(b), avg_launch_us=round(dur * 1e6, 2))
@@ -340,7 +358,7 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     [2e3, 2e5] (structured squares split into triangles, jitter 0.1h), N uniform
     in {2..6}, Ez = sin(m pi x) sin(n pi y) with m, n uniform in {1..4}, all
     drawn from seed 2; dt = 0.5 h_min / (N+1)^2 per instance. The ensemble
-    (one persistent launch per RK4 stage for all instances, the step replayed
+    (one launch per order and RK4 stage for all instances, the step replayed
     from a HIP graph, tilings autotuned per order) against the same steps run
     instance after instance with the per-order kernels."""
     rng = np.random.default_rng(seed)
@@ -386,6 +404,7 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
                                     f"VALU, {64 * (4 // w)} cells per workgroup")
                                 for N, w in sorted(wt_by_order.items())},
                work_items=info["num_items"], grid=info["grid"], lds_bytes=info["lds_bytes"],
+               nontemporal_streams=ens.streaming,
                ensemble=dict(value=v_e, ms_per_step=ms_e / steps),
                sequential=dict(value=v_s, ms_per_step=ms_s / steps),
                speedup=round(v_e / v_s, 3), finite=finite, setup_s=round(t_setup, 1),
@@ -393,6 +412,32 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     print(f"# ensemble of {n_inst} Maxwell instances ({dofs / 1e6:.1f} M DOFs): {v_e / 1e9:.2f} G DOF-stage/s "
           f"vs sequential {v_s / 1e9:.2f} ({out['speedup']}x)", file=sys.stderr, flush=True)
     del ens
+    return out
+
+
+def hbm_resident(dgc, ctx, steps, warmup):
+    """Low-order FP64 RK4 with every vector in HBM: 2048 x 2048 periodic
+    squares (K = 8,388,608, jitter 0.05h, seed 1), advection (a = (1, 0.5),
+    Gaussian pulse) and Maxwell TM (Ez = sin(pi x) sin(pi y)) at N = 1, 2, 3.
+    The smallest working set (advection N = 1: 5 vectors of 201 MB plus
+    369 MB of geometry and connectivity) is 5.4x the 256 MB Infinity Cache,
+    so every stage streams from HBM; the automatic cache policy runs the
+    nontemporal-stream kernels."""
+    hm = dgc.Mesh.structured(2048, 2048, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    out = []
+    for physics in ("advection", "maxwell"):
+        for N in (1, 2, 3):
+            r = run_order(dgc, ctx, hm, N, steps, warmup, prof_steps=3, physics=physics)
+            roof = roofline_for(r)
+            out.append(dict(physics=physics, N=N, K=r["K"], value=r["value"], finite=r["finite"],
+                            stage_us=[round(v * 1e3, 1) for v in r["stage_ms"]],
+                            step_GBs=round(sum(stage_bytes(r["K"], r["Np"], r["F"], s) for s in (1, 2, 3, 4))
+                                           / (sum(r["stage_ms"]) * 1e-3) / 1e9, 1),
+                            roofline=roof))
+            print(f"# HBM-resident {physics} K={r['K']} N={N}: {r['value'] / 1e9:.2f} G DOF-stage/s, "
+                  f"stage-2 {roof['achieved']} GB/s ({roof['frac']:.1%}), whole step {out[-1]['step_GBs']} GB/s",
+                  file=sys.stderr, flush=True)
+            del r
     return out
 
 
@@ -405,7 +450,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hbm", action="store_true")
     ap.add_argument("--hbm-only", action="store_true",
-                    help="run only the HBM-resident case (for counter collection)")
+                    help="run only the HBM-resident N = 3 case (for counter collection)")
+    ap.add_argument("--hbm-physics", default="advection", choices=("advection", "maxwell"))
     ap.add_argument("--calib", action="store_true",
                     help="also run dgc_axpy over 2^27 doubles: a kernel of known traffic "
                          "(reads 2 GiB, writes 1 GiB) to scale HBM byte counters")
@@ -435,9 +481,9 @@ def main():
         print("# calibration: 3 x dgc_axpy over 2^27 doubles done", file=sys.stderr, flush=True)
         del x, y, o
     if args.hbm_only:
-        hm = dgc.Mesh.structured(1024, 1024, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
-        r = run_order(dgc, ctx, hm, 3, args.steps, args.warmup)
-        print(json.dumps(dict(workload="advection2d_fp64_rk4 K=2097152 N=3 (HBM-resident)",
+        hm = dgc.Mesh.structured(2048, 2048, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+        r = run_order(dgc, ctx, hm, 3, args.steps, args.warmup, physics=args.hbm_physics)
+        print(json.dumps(dict(workload=f"{args.hbm_physics} fp64 rk4 K={r['K']} N=3 (HBM-resident)",
                               value=r["value"], roofline=roofline_for(r),
                               stage_us=[round(v * 1e3, 2) for v in r["stage_ms"]])))
         return
@@ -472,13 +518,7 @@ def main():
 
     extra = {}
     if not args.no_hbm:
-        hm = dgc.Mesh.structured(1024, 1024, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
-        r = run_order(dgc, ctx, hm, 3, max(10, args.steps // 10), 2)
-        extra["hbm_resident"] = dict(K=r["K"], N=3, value=r["value"], roofline=roofline_for(r),
-                                     note="2.1M cells x Np=10: 168 MB per state vector, above the "
-                                          "256 MB Infinity Cache with the 5 RK buffers")
-        print(f"# HBM-resident K={r['K']} N=3: {r['value'] / 1e9:.2f} G DOF-stage/s, "
-              f"stage-2 {roof and roofline_for(r)['achieved']} GB/s", file=sys.stderr, flush=True)
+        extra["hbm_resident"] = hbm_resident(dgc, ctx, max(10, args.steps // 20), 2)
 
     if not args.no_ipdg:
         full = args.ipdg_full

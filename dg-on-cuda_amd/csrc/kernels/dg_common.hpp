@@ -42,4 +42,29 @@ constexpr int kBoundaryCode = 3;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Bijective workgroup renumbering: consecutive hardware dispatch slots go to
+// the 8 XCDs in turn; give XCD x the contiguous range of logical groups
+// [x*per + min(x, rem), ...), so neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_group(int b, int G) {
+  constexpr int X = 8;
+  const int x = b % X, i = b / X, per = G / X, rem = G % X;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+// Accesses of the RK streams that a stage kernel reads or writes once (the
+// accumulator, k3, the step's initial state). With NT they carry the
+// nontemporal hint: used when the RK working set exceeds the last-level
+// cache, so that these streams do not displace the stage input, whose node
+// values are read again by the neighbouring cells.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_once(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_once(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 }  // namespace dgc

@@ -1,103 +1,14 @@
-// Batched ensemble: many independent Maxwell TM instances (own mesh, size
-// and order each) advance together, one RK4 stage at a time.
-//
-// One launch per order and stage covers every instance of that order: its
-// work list holds one item per (instance, group of 4/WT tiles of 64 cells),
-// the items of an instance contiguous, and each XCD takes a contiguous range
-// of the list, so the neighbour traces a tile reads were usually loaded into
-// the same L2 by the tile before it. Per item the workgroup runs the device
-// code of the per-order kernels (stage_tile, dg_stage_tile.hpp) with that
-// instance's stage parameters. Kernels of different orders share nothing, so
-// the step graph runs the orders as independent branches.
-// No reference counterpart: the reference runs one mesh per process.
-#include <algorithm>
-#include <cstdint>
-
-#include "dg_stage_mfma.hpp"
-#include "dg_stage_tile.hpp"
+// Launcher of the batched-ensemble stage kernels (dg_ensemble.hpp).
+#include "dg_ensemble.hpp"
 
 namespace dgc {
 
-namespace {
+bool ensemble_supports(int order, int wt) { return ens_detail::ensemble_fn<false>(order, wt, 1) != nullptr; }
 
-// The per-(instance, stage) parameter records are read through the constant
-// address space: like kernel arguments they are then invariant scalar loads
-// that the compiler issues next to their uses instead of holding them in
-// registers for the whole item.
-using ConstParams = const StageParams __attribute__((address_space(4)));
-
-// One workgroup per item, items mapped so that each XCD takes a contiguous
-// range. (A persistent variant looping over items let the compiler hoist
-// per-item address arithmetic out of the loop: 191 -> 256 VGPRs plus AGPR
-// spills at N = 3, half the occupancy.)
-template <int N, int WT, int STAGE>
-__global__ __launch_bounds__(kBlock) void ensemble_stage_kernel(const StageParams* __restrict__ params,
-                                                                const int2* __restrict__ items) {
-  __shared__ double lds[stage_lds_doubles(N, kMaxwellTM, WT)];
-  ConstParams* base = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(params));
-  const int2 item = items[xcd_group(blockIdx.x, gridDim.x)];  // (instance, tile group)
-  const StageParams& p = *(const StageParams*)(base + (uniform(item.x) * 4 + STAGE - 1));
-  stage_tile<N, kMaxwellTM, STAGE, WT>(p, uniform(item.y), lds);
-}
-
-// The FP64 matrix-core variant (tiling 0): one 64-cell tile per item.
-template <int N, int STAGE>
-__global__ __launch_bounds__(kBlock) void ensemble_mfma_kernel(const StageParams* __restrict__ params,
-                                                               const int2* __restrict__ items) {
-  __shared__ __align__(16) unsigned char lds[mfma_lds_bytes<double, N, kMaxwellTM>()];
-  ConstParams* base = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(params));
-  const int2 item = items[xcd_group(blockIdx.x, gridDim.x)];
-  const StageParams& p = *(const StageParams*)(base + (uniform(item.x) * 4 + STAGE - 1));
-  stage_tile_mfma<double, N, kMaxwellTM, STAGE>(p, uniform(item.y), lds);
-}
-
-using EnsFn = void (*)(const StageParams*, const int2*);
-
-template <int N, int WT>
-EnsFn stage_of(int stage) {
-  if constexpr (WT == kMfmaTiling) {
-    return stage == 1 ? ensemble_mfma_kernel<N, 1>
-         : stage == 2 ? ensemble_mfma_kernel<N, 2>
-         : stage == 3 ? ensemble_mfma_kernel<N, 3>
-                      : ensemble_mfma_kernel<N, 4>;
-  } else {
-    return stage == 1 ? ensemble_stage_kernel<N, WT, 1>
-         : stage == 2 ? ensemble_stage_kernel<N, WT, 2>
-         : stage == 3 ? ensemble_stage_kernel<N, WT, 3>
-                      : ensemble_stage_kernel<N, WT, 4>;
-  }
-}
-
-EnsFn ensemble_fn(int order, int wt, int stage) {
-  switch (order * 8 + wt) {
-    case 1 * 8 + 2: return stage_of<1, 2>(stage);
-    case 1 * 8 + 4: return stage_of<1, 4>(stage);
-    case 2 * 8 + 2: return stage_of<2, 2>(stage);
-    case 2 * 8 + 4: return stage_of<2, 4>(stage);
-    case 3 * 8 + 2: return stage_of<3, 2>(stage);
-    case 3 * 8 + 4: return stage_of<3, 4>(stage);
-    case 4 * 8 + 4: return stage_of<4, 4>(stage);
-    case 5 * 8 + 4: return stage_of<5, 4>(stage);
-    case 6 * 8 + 4: return stage_of<6, 4>(stage);
-    case 7 * 8 + 4: return stage_of<7, 4>(stage);
-    case 1 * 8 + kMfmaTiling: return stage_of<1, kMfmaTiling>(stage);
-    case 2 * 8 + kMfmaTiling: return stage_of<2, kMfmaTiling>(stage);
-    case 3 * 8 + kMfmaTiling: return stage_of<3, kMfmaTiling>(stage);
-    case 4 * 8 + kMfmaTiling: return stage_of<4, kMfmaTiling>(stage);
-    case 5 * 8 + kMfmaTiling: return stage_of<5, kMfmaTiling>(stage);
-    case 6 * 8 + kMfmaTiling: return stage_of<6, kMfmaTiling>(stage);
-    case 7 * 8 + kMfmaTiling: return stage_of<7, kMfmaTiling>(stage);
-    default: return nullptr;
-  }
-}
-
-}  // namespace
-
-bool ensemble_supports(int order, int wt) { return ensemble_fn(order, wt, 1) != nullptr; }
-
-hipError_t launch_ensemble_stage(int order, int wt, int stage, const StageParams* params, const int2* items,
-                                 int nitems, hipStream_t s) {
-  EnsFn fn = ensemble_fn(order, wt, stage);
+hipError_t launch_ensemble_stage(int order, int wt, bool streaming, int stage, const StageParams* params,
+                                 const int2* items, int nitems, hipStream_t s) {
+  ens_detail::EnsFn fn = streaming ? ens_detail::ensemble_fn_streaming(order, wt, stage)
+                                   : ens_detail::ensemble_fn<false>(order, wt, stage);
   if (fn == nullptr || stage < 1 || stage > 4 || nitems < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(nitems), dim3(kBlock), 0, s, params, items);
   return hipGetLastError();

@@ -64,7 +64,22 @@ struct StageParams {
   // blocks opD serve the FP64 matrix-core variant (tiling code 0).
   const float* __restrict__ opF;
   const double* __restrict__ opD;
+  // Host-side kernel choice, not read by device code: 1 selects the variant
+  // with nontemporal accumulator streams.
+  int streaming;
 };
+
+using StageFn = void (*)(const StageParams);
+
+// MI355X last-level (Infinity) cache. The RK stage kernels use nontemporal
+// accumulator streams when a step's working set exceeds it.
+constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
+
+// Bytes one RK4 step touches: the state and the four work vectors of F
+// fields, plus the 4 metric terms and 3 connectivity codes per cell.
+inline size_t rk_working_set_bytes(int K, int Np, int F) {
+  return size_t(K) * (5 * size_t(F) * Np * sizeof(double) + 4 * sizeof(double) + 3 * sizeof(int));
+}
 
 // Tiling code of the FP64 matrix-core variant of the stage kernels.
 constexpr int kMfmaTiling = 0;
@@ -82,9 +97,10 @@ hipError_t launch_stage_mfma(int order, int physics, int stage, const StageParam
 // of one order. params: 4 stage parameter records per instance (stages 1..4);
 // items: (instance, tile group) pairs of those instances, the items of an
 // instance contiguous.
+// streaming: the variant with nontemporal accumulator streams.
 bool ensemble_supports(int order, int wt);
-hipError_t launch_ensemble_stage(int order, int wt, int stage, const StageParams* params, const int2* items,
-                                 int nitems, hipStream_t s);
+hipError_t launch_ensemble_stage(int order, int wt, bool streaming, int stage, const StageParams* params,
+                                 const int2* items, int nitems, hipStream_t s);
 
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }

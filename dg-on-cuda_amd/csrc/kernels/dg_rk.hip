@@ -1,67 +1,18 @@
-// Per-order launchers of the fused FP64 stage (the device code is in
-// dg_stage_tile.hpp, shared with the ensemble kernel).
-#include "dg_stage_tile.hpp"
+// Launcher of the fused FP64 stage kernels (dg_rk_kernels.hpp; device code
+// in dg_stage_tile.hpp, shared with the ensemble kernel).
+#include "dg_rk_kernels.hpp"
 
 namespace dgc {
 
-template <int N, int PHYS, int STAGE, int WT>
-__global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
-  __shared__ double lds[stage_lds_doubles(N, PHYS, WT)];
-  stage_tile<N, PHYS, STAGE, WT>(p, xcd_group(blockIdx.x, gridDim.x), lds);
-}
-
 namespace {
 
-using StageFn = void (*)(const StageParams);
-
-template <int N, int PHYS, int WT>
-constexpr StageFn stage_fn(int stage) {
-  return stage == 0 ? dg_stage_kernel<N, PHYS, 0, WT>
-       : stage == 1 ? dg_stage_kernel<N, PHYS, 1, WT>
-       : stage == 2 ? dg_stage_kernel<N, PHYS, 2, WT>
-       : stage == 3 ? dg_stage_kernel<N, PHYS, 3, WT>
-                    : dg_stage_kernel<N, PHYS, 4, WT>;
-}
-
-// A variant is built only if its LDS (all tiles of the workgroup) stays
-// within 80 KB, so that at least two workgroups share a CU; larger requests
-// fall back to one tile per workgroup (WT = 4).
-template <int N, int PHYS, int WT>
-constexpr bool fits() {
-  constexpr int Np = Tri<N>::Np, NF = 3 * Tri<N>::Nfp, F = PHYS == kMaxwellTM ? 3 : 1;
-  return WT == 4 || (kWaves / WT) * F * (Np + NF) * kTile * 8 <= 80 * 1024;
-}
-
-template <int N, int PHYS>
-StageFn pick_wt(int wt, int stage) {
-  if constexpr (fits<N, PHYS, 1>()) {
-    if (wt == 1) return stage_fn<N, PHYS, 1>(stage);
-  }
-  if constexpr (fits<N, PHYS, 2>()) {
-    if (wt == 2) return stage_fn<N, PHYS, 2>(stage);
-  }
-  return stage_fn<N, PHYS, 4>(stage);
-}
+using rk_detail::fits;
 
 template <int N, int PHYS>
 int effective_wt(int wt) {
   if (wt == 1 && fits<N, PHYS, 1>()) return 1;
   if (wt == 2 && fits<N, PHYS, 2>()) return 2;
   return 4;
-}
-
-template <int PHYS>
-StageFn pick(int order, int wt, int stage) {
-  switch (order) {
-    case 1: return pick_wt<1, PHYS>(wt, stage);
-    case 2: return pick_wt<2, PHYS>(wt, stage);
-    case 3: return pick_wt<3, PHYS>(wt, stage);
-    case 4: return pick_wt<4, PHYS>(wt, stage);
-    case 5: return pick_wt<5, PHYS>(wt, stage);
-    case 6: return pick_wt<6, PHYS>(wt, stage);
-    case 7: return pick_wt<7, PHYS>(wt, stage);
-    default: return nullptr;
-  }
 }
 
 }  // namespace
@@ -97,7 +48,8 @@ hipError_t launch_stage(int order, int physics, int stage, int wt, const StagePa
   }
   if (stage < 0 || stage > 4 || (wt != 1 && wt != 2 && wt != 4)) return hipErrorInvalidValue;
   wt = supported_waves_per_tile(order, physics, wt);
-  StageFn fn = physics == kMaxwellTM ? pick<kMaxwellTM>(order, wt, stage) : pick<kAdvection>(order, wt, stage);
+  StageFn fn = p.streaming ? rk_detail::pick_streaming(order, physics, wt, stage)
+                           : rk_detail::pick<false>(order, physics, wt, stage);
   if (fn == nullptr) return hipErrorInvalidValue;
   const int tiles = (p.K + kTile - 1) / kTile;
   const int tpw = kWaves / wt;

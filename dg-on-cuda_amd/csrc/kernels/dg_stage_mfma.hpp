@@ -75,7 +75,8 @@ constexpr int mfma_lds_bytes() {
   return F * (Np + NF) * kTile * 8 + (mfma_operands_in_lds<PHYS>() ? MB * (2 * KS + KL) * 64 * int(sizeof(CT)) : 0);
 }
 
-template <typename CT, int N, int PHYS, int STAGE>
+// NT: nontemporal hints on the accumulator streams (ld_once / st_once).
+template <typename CT, int N, int PHYS, int STAGE, bool NT>
 __device__ __forceinline__ void stage_tile_mfma(const StageParams& p, int tile, unsigned char* lds) {
   using namespace mfma_detail;
   using T = Tri<N>;
@@ -221,14 +222,14 @@ __device__ __forceinline__ void stage_tile_mfma(const StageParams& p, int tile, 
         o1[f][r] = o2[f][r] = o3[f][r] = 0.0;
         if (ovalid && j < Np) {
           if (STAGE == 2) {
-            o1[f][r] = p.acc[f][idx];
-            o2[f][r] = p.u[f][idx];
+            o1[f][r] = ld_once<NT>(p.acc[f] + idx);
+            o2[f][r] = ld_once<NT>(p.u[f] + idx);
           } else if (STAGE == 3) {
-            o2[f][r] = p.u[f][idx];
+            o2[f][r] = ld_once<NT>(p.u[f] + idx);
           } else if (STAGE == 4) {
-            o1[f][r] = p.acc[f][idx];
-            o2[f][r] = p.u[f][idx];
-            o3[f][r] = p.k3[f][idx];
+            o1[f][r] = ld_once<NT>(p.acc[f] + idx);
+            o2[f][r] = ld_once<NT>(p.u[f] + idx);
+            o3[f][r] = ld_once<NT>(p.k3[f] + idx);
           }
         }
       }
@@ -276,22 +277,67 @@ __device__ __forceinline__ void stage_tile_mfma(const StageParams& p, int tile, 
         if (STAGE == 0) {
           p.out[f][idx] = res[f];
         } else if (STAGE == 1) {
-          p.acc[f][idx] = res[f];
+          st_once<NT>(p.acc[f] + idx, res[f]);
           p.out[f][idx] = p.c1 * res[f] + us[f][j][q];
         } else if (STAGE == 2) {
-          p.acc[f][idx] = 2.0 * res[f] + o1[f][r];
+          st_once<NT>(p.acc[f] + idx, 2.0 * res[f] + o1[f][r]);
           p.out[f][idx] = p.c1 * res[f] + o2[f][r];
         } else if (STAGE == 3) {
-          p.k3[f][idx] = res[f];
+          st_once<NT>(p.k3[f] + idx, res[f]);
           p.out[f][idx] = p.c1 * res[f] + o2[f][r];
         } else {
           const double w1 = 2.0 * o3[f][r] + res[f];
           const double w2 = p.c1 * o1[f][r] + o2[f][r];
-          p.u[f][idx] = p.c1 * w1 + w2;
+          st_once<NT>(p.u[f] + idx, p.c1 * w1 + w2);
         }
       }
     }
   }
 }
+
+template <typename CT, int N, int PHYS, int STAGE, bool NT>
+__global__ __launch_bounds__(kBlock) void dg_stage_mfma_kernel(const StageParams p) {
+  __shared__ __align__(16) unsigned char lds[mfma_lds_bytes<CT, N, PHYS>()];
+  stage_tile_mfma<CT, N, PHYS, STAGE, NT>(p, xcd_group(blockIdx.x, gridDim.x), lds);
+}
+
+namespace mfma_detail {
+
+template <typename CT, int N, int PHYS, bool NT>
+StageFn stage_fn(int stage) {
+  return stage == 0 ? dg_stage_mfma_kernel<CT, N, PHYS, 0, NT>
+       : stage == 1 ? dg_stage_mfma_kernel<CT, N, PHYS, 1, NT>
+       : stage == 2 ? dg_stage_mfma_kernel<CT, N, PHYS, 2, NT>
+       : stage == 3 ? dg_stage_mfma_kernel<CT, N, PHYS, 3, NT>
+                    : dg_stage_mfma_kernel<CT, N, PHYS, 4, NT>;
+}
+
+template <typename CT, int PHYS, bool NT>
+StageFn pick_order(int order, int stage) {
+  switch (order) {
+    case 1: return stage_fn<CT, 1, PHYS, NT>(stage);
+    case 2: return stage_fn<CT, 2, PHYS, NT>(stage);
+    case 3: return stage_fn<CT, 3, PHYS, NT>(stage);
+    case 4: return stage_fn<CT, 4, PHYS, NT>(stage);
+    case 5: return stage_fn<CT, 5, PHYS, NT>(stage);
+    case 6: return stage_fn<CT, 6, PHYS, NT>(stage);
+    case 7: return stage_fn<CT, 7, PHYS, NT>(stage);
+    default: return nullptr;
+  }
+}
+
+// Kernel of the FP32-contraction form (f32, advection only) or of the FP64
+// matrix-core variant (both operators); nullptr if not built.
+template <bool NT>
+StageFn pick(int order, int physics, bool f32, int stage) {
+  if (f32) return physics == kAdvection ? pick_order<float, kAdvection, NT>(order, stage) : nullptr;
+  return physics == kMaxwellTM ? pick_order<double, kMaxwellTM, NT>(order, stage)
+                               : pick_order<double, kAdvection, NT>(order, stage);
+}
+
+// The nontemporal table (dg_rk_mfma_nt.hip).
+StageFn pick_streaming(int order, int physics, bool f32, int stage);
+
+}  // namespace mfma_detail
 
 }  // namespace dgc

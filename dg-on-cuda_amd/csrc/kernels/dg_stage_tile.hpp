@@ -29,19 +29,11 @@
 
 namespace dgc {
 
-// Bijective workgroup renumbering: consecutive hardware dispatch slots go to
-// the 8 XCDs in turn; give XCD x the contiguous range of logical groups
-// [x*per + min(x, rem), ...).
-__device__ __forceinline__ int xcd_group(int b, int G) {
-  constexpr int X = 8;
-  const int x = b % X, i = b / X, per = G / X, rem = G % X;
-  return x * per + (x < rem ? x : rem) + i;
-}
-
 // One workgroup's share of a stage: the TPW = 4/WT tiles of tile group
 // `group` (cells group*64*TPW ...), with `lds` holding stage_lds_doubles()
 // doubles. Used by the per-order kernels below and by the ensemble kernel.
-template <int N, int PHYS, int STAGE, int WT>
+// NT: nontemporal hints on the accumulator streams (ld_once / st_once).
+template <int N, int PHYS, int STAGE, int WT, bool NT>
 __device__ __forceinline__ void stage_tile(const StageParams& p, int group, double* lds) {
   using T = Tri<N>;
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
@@ -185,14 +177,14 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
       o1[f][i] = o2[f][i] = o3[f][i] = 0.0;
       if (ok) {
         if (STAGE == 2) {
-          o1[f][i] = p.acc[f][idx];
-          o2[f][i] = p.u[f][idx];
+          o1[f][i] = ld_once<NT>(p.acc[f] + idx);
+          o2[f][i] = ld_once<NT>(p.u[f] + idx);
         } else if (STAGE == 3) {
-          o2[f][i] = p.u[f][idx];
+          o2[f][i] = ld_once<NT>(p.u[f] + idx);
         } else if (STAGE == 4) {
-          o1[f][i] = p.acc[f][idx];
-          o2[f][i] = p.u[f][idx];
-          o3[f][i] = p.k3[f][idx];
+          o1[f][i] = ld_once<NT>(p.acc[f] + idx);
+          o2[f][i] = ld_once<NT>(p.u[f] + idx);
+          o3[f][i] = ld_once<NT>(p.k3[f] + idx);
         }
       }
     }
@@ -264,18 +256,18 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
       if (STAGE == 0) {
         p.out[f][idx] = r;
       } else if (STAGE == 1) {
-        p.acc[f][idx] = r;
+        st_once<NT>(p.acc[f] + idx, r);
         p.out[f][idx] = p.c1 * r + us[tw][f][j][lane];
       } else if (STAGE == 2) {
-        p.acc[f][idx] = 2.0 * r + o1[f][i];
+        st_once<NT>(p.acc[f] + idx, 2.0 * r + o1[f][i]);
         p.out[f][idx] = p.c1 * r + o2[f][i];
       } else if (STAGE == 3) {
-        p.k3[f][idx] = r;
+        st_once<NT>(p.k3[f] + idx, r);
         p.out[f][idx] = p.c1 * r + o2[f][i];
       } else {
         const double w1 = 2.0 * o3[f][i] + r;
         const double w2 = p.c1 * o1[f][i] + o2[f][i];
-        p.u[f][idx] = p.c1 * w1 + w2;
+        st_once<NT>(p.u[f] + idx, p.c1 * w1 + w2);
       }
     }
   }

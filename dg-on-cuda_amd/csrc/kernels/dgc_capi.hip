@@ -47,6 +47,7 @@ struct dgc_op_s {
   float* opF = nullptr;   // matrix-core operand blocks of Dr, Ds, LIFT (advection)
   double* opD = nullptr;
   std::vector<double> hDr, hDs, hL;  // host copies of the operators
+  int cache_policy = DGC_CACHE_AUTO;
   double* tclock = nullptr;
   // cached graph of one full RK4 step
   hipGraphExec_t graph = nullptr;
@@ -362,6 +363,12 @@ int dgc_op_destroy(dgc_op_t op) {
 
 int dgc_op_num_fields(dgc_op_t op) { return op ? op->F : 0; }
 
+static int op_streaming(const dgc_op_s* op) {
+  if (op->cache_policy == DGC_CACHE_AUTO)
+    return dgc::rk_working_set_bytes(op->disc->K, op->disc->Np, op->F) > dgc::kLastLevelCacheBytes ? 1 : 0;
+  return op->cache_policy == DGC_CACHE_STREAMING ? 1 : 0;
+}
+
 static dgc::StageParams base_params(const dgc_op_s* op) {
   dgc::StageParams p{};
   const dgc_disc_s* d = op->disc;
@@ -379,6 +386,7 @@ static dgc::StageParams base_params(const dgc_op_s* op) {
   p.tclock = nullptr;
   p.opF = op->precision == 1 ? op->opF : nullptr;
   p.opD = op->opD;
+  p.streaming = op_streaming(op);
   return p;
 }
 
@@ -631,6 +639,20 @@ int dgc_op_set_precision(dgc_op_t op, int precision) {
 
 int dgc_op_precision(dgc_op_t op) { return op ? op->precision : -1; }
 
+int dgc_op_set_cache_policy(dgc_op_t op, int policy) {
+  DGC_REQUIRE(op != nullptr, "dgc_op_set_cache_policy: null operator");
+  DGC_REQUIRE(policy == DGC_CACHE_AUTO || policy == DGC_CACHE_TEMPORAL || policy == DGC_CACHE_STREAMING,
+              "dgc_op_set_cache_policy: policy must be DGC_CACHE_AUTO, DGC_CACHE_TEMPORAL or DGC_CACHE_STREAMING");
+  op->cache_policy = policy;
+  if (op->graph) {
+    (void)hipGraphExecDestroy(op->graph);
+    op->graph = nullptr;
+  }
+  return DGC_OK;
+}
+
+int dgc_op_cache_streaming(dgc_op_t op) { return op ? op_streaming(op) : -1; }
+
 // Times the stage-2 kernel (the widest: 5 state passes) of every supported
 // tiling on the caller's buffers and keeps the fastest. Only work vectors are
 // written (stage 2 writes wk0 and wk2, which the next step's stage 1
@@ -749,6 +771,7 @@ int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, doubl
 // Batched ensemble of independent Maxwell TM instances.
 struct EnsembleGroup {
   int order = 0, wt = 4, offset = 0, nitems = 0;
+  bool streaming = false;  // nontemporal accumulator streams
   hipStream_t stream = nullptr;  // branch of the step graph
 };
 
@@ -762,8 +785,16 @@ struct dgc_ensemble_s {
   dgc::StageParams* d_params = nullptr;  // [instance][stage 1..4]
   int2* d_items = nullptr;
   int nitems = 0;
+  int cache_policy = DGC_CACHE_AUTO;
   hipGraphExec_t graph = nullptr;
 };
+
+static bool ensemble_streaming(const dgc_ensemble_s* e) {
+  if (e->cache_policy != DGC_CACHE_AUTO) return e->cache_policy == DGC_CACHE_STREAMING;
+  size_t ws = 0;
+  for (const dgc_op_s* op : e->ops) ws += dgc::rk_working_set_bytes(op->disc->K, op->disc->Np, op->F);
+  return ws > dgc::kLastLevelCacheBytes;
+}
 
 static void ensemble_release(dgc_ensemble_s* e) {
   if (e->graph) (void)hipGraphExecDestroy(e->graph);
@@ -815,6 +846,7 @@ static int ensemble_build(dgc_ensemble_s* e) {
     if (e->groups.empty() || e->groups.back().order != order) {
       EnsembleGroup g;
       g.order = order;
+      g.streaming = ensemble_streaming(e);
       g.wt = e->wt[i];  // the largest instance of the order sets the group's tiling
       g.offset = int(items.size());
       e->groups.push_back(g);
@@ -889,7 +921,7 @@ int dgc_ensemble_autotune(dgc_ensemble_t e, int reps) {
       if (e->ops[i]->disc->order == order && (big < 0 || e->ops[i]->disc->K > e->ops[big]->disc->K)) big = i;
     if (big < 0) continue;
     dgc_op_s* op = e->ops[big];
-    float ms[3];
+    float ms[4];
     int rc = dgc_op_autotune(op, e->state[big].data(), size_t(op->disc->K) * op->disc->Np, e->work[big].data(),
                              reps, ms);
     if (rc != DGC_OK) return rc;
@@ -915,6 +947,16 @@ int dgc_ensemble_info(dgc_ensemble_t e, int* wt, int* num_items, int* grid, size
   return DGC_OK;
 }
 
+int dgc_ensemble_set_cache_policy(dgc_ensemble_t e, int policy) {
+  DGC_REQUIRE(e != nullptr, "dgc_ensemble_set_cache_policy: null ensemble");
+  DGC_REQUIRE(policy == DGC_CACHE_AUTO || policy == DGC_CACHE_TEMPORAL || policy == DGC_CACHE_STREAMING,
+              "dgc_ensemble_set_cache_policy: policy must be DGC_CACHE_AUTO, DGC_CACHE_TEMPORAL or DGC_CACHE_STREAMING");
+  e->cache_policy = policy;
+  return ensemble_build(e);
+}
+
+int dgc_ensemble_cache_streaming(dgc_ensemble_t e) { return e ? (ensemble_streaming(e) ? 1 : 0) : -1; }
+
 int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms) {
   DGC_REQUIRE(e && nsteps >= 0, "dgc_ensemble_steps: bad argument");
   hipStream_t st = e->ctx->stream;
@@ -932,8 +974,8 @@ int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms) {
       const EnsembleGroup& gr = e->groups[k];
       le = hipStreamWaitEvent(gr.stream, evs[0], 0);
       for (int s = 1; s <= 4 && le == hipSuccess; ++s)
-        le = dgc::launch_ensemble_stage(gr.order, gr.wt, s, e->d_params, e->d_items + gr.offset, gr.nitems,
-                                        gr.stream);
+        le = dgc::launch_ensemble_stage(gr.order, gr.wt, gr.streaming, s, e->d_params, e->d_items + gr.offset,
+                                        gr.nitems, gr.stream);
       if (le == hipSuccess) le = hipEventRecord(evs[k + 1], gr.stream);
       if (le == hipSuccess) le = hipStreamWaitEvent(st, evs[k + 1], 0);
     }

@@ -88,6 +88,8 @@ def _declare(L):
     d("dgc_op_waves_per_tile", [_vp])
     d("dgc_op_set_precision", [_vp, C.c_int])
     d("dgc_op_precision", [_vp])
+    d("dgc_op_set_cache_policy", [_vp, C.c_int])
+    d("dgc_op_cache_streaming", [_vp])
     d("dgc_op_autotune", [_vp, C.POINTER(_vp), C.c_size_t, C.POINTER(_vp), C.c_int, C.POINTER(C.c_float)])
     for name, args in _EXTRA_DECLS:
         d(name, args)
@@ -410,9 +412,21 @@ class _Operator:
     def precision(self, value: str):
         check(lib().dgc_op_set_precision(self._h, {"fp64": 0, "fp32-mfma": 1}[value]))
 
+    CACHE_POLICIES = {"auto": 0, "temporal": 1, "streaming": 2}
+
+    @property
+    def streaming(self) -> bool:
+        """True if the stage kernels run with nontemporal accumulator streams."""
+        return lib().dgc_op_cache_streaming(self._h) == 1
+
+    def set_cache_policy(self, policy: str):
+        """'auto' (stream when a step's working set exceeds the 256 MB
+        last-level cache), 'temporal' or 'streaming'."""
+        check(lib().dgc_op_set_cache_policy(self._h, self.CACHE_POLICIES[policy]))
+
     def autotune(self, state, work, reps=20) -> dict:
         """Time every built tiling on these buffers and keep the fastest (0 is the
-        advection operator's FP64 matrix-core variant)."""
+        FP64 matrix-core variant)."""
         ms = (C.c_float * 4)()
         check(lib().dgc_op_autotune(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
                                     reps, ms))

@@ -25,6 +25,8 @@ def _declare():
     d("dgc_ensemble_autotune", [_vp, C.c_int])
     d("dgc_ensemble_info", [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                             C.POINTER(C.c_size_t)])
+    d("dgc_ensemble_set_cache_policy", [_vp, C.c_int])
+    d("dgc_ensemble_cache_streaming", [_vp])
     d("dgc_ensemble_steps", [_vp, C.c_int, C.POINTER(C.c_float)])
     d("dgc_ensemble_steps_sequential", [_vp, C.c_int, C.POINTER(C.c_float)])
     L._ens_declared = True
@@ -32,8 +34,9 @@ def _declare():
 
 
 class Ensemble:
-    """Independent Maxwell TM instances advanced together: one persistent
-    launch per RK4 stage for all of them, the step replayed from a HIP graph.
+    """Independent Maxwell TM instances advanced together: one launch per
+    order and RK4 stage for all instances of that order, the step replayed
+    from a HIP graph.
 
     ops[i]: MaxwellTM operator, states[i]: its 3 field DeviceArrays,
     works[i]: its work vectors (op.new_work()), dts[i]: its time step."""
@@ -61,6 +64,16 @@ class Ensemble:
         lds = C.c_size_t()
         check(lib().dgc_ensemble_info(self._h, wt, C.byref(ni), C.byref(g), C.byref(lds)))
         return dict(waves_per_tile=list(wt), num_items=ni.value, grid=g.value, lds_bytes=lds.value)
+
+    @property
+    def streaming(self) -> bool:
+        """True if the stage kernels run with nontemporal accumulator streams."""
+        return lib().dgc_ensemble_cache_streaming(self._h) == 1
+
+    def set_cache_policy(self, policy: str):
+        """'auto' (stream when all instances' working sets together exceed the
+        256 MB last-level cache), 'temporal' or 'streaming'."""
+        check(lib().dgc_ensemble_set_cache_policy(self._h, {"auto": 0, "temporal": 1, "streaming": 2}[policy]))
 
     def steps(self, nsteps: int) -> float:
         ms = C.c_float()

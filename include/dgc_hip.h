@@ -134,6 +134,18 @@ int dgc_op_waves_per_tile(dgc_op_t op);
  * Form 1 is built for the advection operator. */
 int dgc_op_set_precision(dgc_op_t op, int precision);
 int dgc_op_precision(dgc_op_t op);
+/* Cache policy of the RK stage kernels (no reference counterpart). The RK4
+ * accumulator, k3 and the step's initial state are read or written once per
+ * stage; with the streaming policy those accesses carry the nontemporal
+ * hint, so they do not displace the stage input, whose node values the
+ * neighbouring cells read again. DGC_CACHE_AUTO (default) streams when a
+ * step's working set (state + 4 work vectors + 44 bytes of geometry per
+ * cell) exceeds MI355X's 256 MB last-level cache. Results are identical
+ * under every policy. dgc_op_cache_streaming: 1 if the streaming kernels
+ * run, 0 if not, -1 for a null operator. */
+enum { DGC_CACHE_AUTO = 0, DGC_CACHE_TEMPORAL = 1, DGC_CACHE_STREAMING = 2 };
+int dgc_op_set_cache_policy(dgc_op_t op, int policy);
+int dgc_op_cache_streaming(dgc_op_t op);
 int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
                     float* ms_out);
 
@@ -313,6 +325,11 @@ int dgc_ensemble_destroy(dgc_ensemble_t e);
 int dgc_ensemble_autotune(dgc_ensemble_t e, int reps);
 /* wt[n]: waves per 64-cell tile per instance; work-list length, launch grid, LDS bytes. */
 int dgc_ensemble_info(dgc_ensemble_t e, int* wt, int* num_items, int* grid, size_t* lds_bytes);
+/* Cache policy of the ensemble's stage kernels (DGC_CACHE_*; AUTO streams
+ * when the working sets of all instances together exceed the last-level
+ * cache). dgc_ensemble_cache_streaming: 1 if the streaming kernels run. */
+int dgc_ensemble_set_cache_policy(dgc_ensemble_t e, int policy);
+int dgc_ensemble_cache_streaming(dgc_ensemble_t e);
 /* nsteps RK4 steps of every instance (graph replays); *ms: HIP-event time. */
 int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms);
 /* Baseline: the same steps instance after instance with the per-instance

@@ -1,5 +1,5 @@
-"""Batched ensemble of Maxwell TM instances on the GPU: one persistent launch
-per RK4 stage for instances of mixed orders and sizes must give each
+"""Batched ensemble of Maxwell TM instances on the GPU: one launch per order
+and RK4 stage for instances of mixed orders and sizes must give each
 instance exactly what its own per-order kernels give, and the reference's
 CPU path through the oracle (FP64, relative L2 <= 1e-12)."""
 import numpy as np
@@ -74,6 +74,31 @@ def test_matrix_core_tiling_in_the_ensemble(ctx):
         got = _download(states[k], disc.K, disc.Np)
         want = _download(ref_state, disc.K, disc.Np)
         assert np.max(np.abs(got - want)) == 0.0, k
+
+
+@pytest.mark.parametrize("tiling", ["default", "mfma"])
+def test_streaming_kernels_in_the_ensemble(ctx, tiling):
+    """The ensemble's nontemporal-stream kernels give bitwise what its
+    temporal kernels give (the small ensemble runs temporal under the
+    automatic policy)."""
+    inst = _instances(ctx, seed=3)
+    if tiling == "mfma":
+        for _, _, op, _, _ in inst:
+            op.waves_per_tile = 0
+    states = [_upload(ctx, u0, d.K, d.Np) for (_, d, _, u0, _) in inst]
+    works = [op.new_work() for (_, _, op, _, _) in inst]
+    ens = dgc_amd.Ensemble(ctx, [i[2] for i in inst], states, works, [i[4] for i in inst])
+    assert not ens.streaming
+    ens.steps(4)
+    temporal = [_download(s, d.K, d.Np) for s, (_, d, _, _, _) in zip(states, inst)]
+    for s, (_, d, _, u0, _) in zip(states, inst):
+        for a, f in zip(s, np.split(u0, 3)):
+            a.upload(c2n(f, d.K, d.Np))
+    ens.set_cache_policy("streaming")
+    assert ens.streaming
+    ens.steps(4)
+    for k, (s, (_, d, _, _, _)) in enumerate(zip(states, inst)):
+        np.testing.assert_array_equal(_download(s, d.K, d.Np), temporal[k], err_msg=str(k))
 
 
 def test_autotuned_and_sequential_modes_agree(ctx):

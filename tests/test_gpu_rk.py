@@ -279,6 +279,35 @@ def test_every_tiling_gives_identical_results(ctx, N):
             assert rel(r, rs[0]) < 1e-14
 
 
+@pytest.mark.parametrize("N", [1, 3, 4])
+def test_cache_policies_give_identical_results(ctx, N):
+    """The streaming kernels differ from the temporal ones only in the
+    nontemporal hint on the accumulator streams: every tiling, both
+    operators and the FP32-contraction form give bitwise the same state.
+    A small mesh selects the temporal kernels under the automatic policy."""
+    mesh = dgc_amd.Mesh.structured(17, 11, 0, 1, 0, 1, jitter=0.05, seed=6, periodic=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    cases = [("adv", wt, "fp64") for wt in (1, 2, 4, 0)] + [("mx", wt, "fp64") for wt in (1, 2, 4, 0)]
+    cases.append(("adv", 4, "fp32-mfma"))
+    for phys, wt, prec in cases:
+        out = {}
+        for policy in ("auto", "temporal", "streaming"):
+            op = (dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO) if phys == "adv"
+                  else dgc_amd.MaxwellTM(disc))
+            op.waves_per_tile = wt
+            op.precision = prec
+            op.set_cache_policy(policy)
+            assert op.streaming == (policy == "streaming")
+            u0 = np.random.default_rng(N).standard_normal(op.F * disc.size)
+            st = [dgc_amd.DeviceArray.from_host(ctx, f) for f in np.split(u0, op.F)]
+            work = op.new_work()
+            op.rk4_steps(st, 0.0, 1e-4, 3, work)  # graph replays
+            op.rk4_step(st, 3e-4, 1e-4, work)     # direct launches
+            out[policy] = np.concatenate([a.download() for a in st])
+        np.testing.assert_array_equal(out["streaming"], out["temporal"], err_msg=f"{phys} wt={wt} {prec}")
+        np.testing.assert_array_equal(out["auto"], out["temporal"], err_msg=f"{phys} wt={wt} {prec}")
+
+
 def test_autotune_leaves_state_untouched(ctx):
     mesh = dgc_amd.Mesh.structured(32, 32, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
     disc = dgc_amd.Discretization(ctx, mesh, 2)
