@@ -109,8 +109,9 @@ def mfma_stats(r):
 # HBM traffic per launch from FETCH_SIZE and WRITE_SIZE counter runs (scaled by the
 # calibration factors x2.000 and x1.000), keyed by (physics, N, K, waves per tile)
 # of the stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
-# profiles/dg_stage_kernel_advection_N3_K2097152_hbm.md hold the runs.
-MEASURED_TRAFFIC = {("advection", 7, 131072, 4): 196.8e6, ("advection", 3, 2097152, 2): 938.1e6}
+# profiles/dg_stage_kernel_N3_K8388608_hbm.md hold the runs.
+MEASURED_TRAFFIC = {("advection", 7, 131072, 4): 196.8e6, ("advection", 3, 8388608, 2): 3737.7e6,
+                    ("maxwell", 3, 8388608, 2): 10754.0e6}
 
 
 def roofline_for(r):
