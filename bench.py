@@ -90,12 +90,17 @@ def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64", 
 
 
 MFMA_PEAK_F32_TFLOPS = 157.3  # MI355X dense FP32 matrix, published
+MFMA_PEAK_F64_TFLOPS = 78.6   # MI355X dense FP64 matrix, published
 
 
-def mfma_stats(r):
-    """Matrix-core rate of the FP32 form's stage-2 kernel: issued flops (the
+def mfma_stats(r, peak=MFMA_PEAK_F32_TFLOPS):
+    """Matrix-core rate of a matrix-core stage-2 kernel: issued flops (the
     16x16x4 blocks, padding included) and useful flops (2 Np^2 + Np 3Nfp
-    multiply-adds per cell for Dr, Ds and LIFT) per launch over its mean duration."""
+    multiply-adds per cell for Dr, Ds and LIFT) per launch over its mean
+    duration. The issued count equals SQ_INSTS_VALU_MFMA_MOPS_F32 (F64) x 512
+    per launch, and the fraction of peak equals SQ_VALU_MFMA_BUSY_CYCLES per
+    SIMD over the kernel's cycles
+    (profiles/dg_stage_mfma_kernel_advection_N4-7_K131072_mfma.md)."""
     N, Np, K = r["N"], r["Np"], r["K"]
     NF = 3 * (N + 1)
     MB, KS, KL = (Np + 15) // 16, (Np + 3) // 4, (NF + 3) // 4
@@ -103,7 +108,7 @@ def mfma_stats(r):
     useful = 2.0 * (2 * Np * Np + Np * NF) * K
     dur = r["tune"]["ms"][r["tune"]["choice"]] * 1e-3
     return dict(issued_tflops=round(issued / dur / 1e12, 3), useful_tflops=round(useful / dur / 1e12, 3),
-                peak_tflops=MFMA_PEAK_F32_TFLOPS, frac_issued=round(issued / dur / 1e12 / MFMA_PEAK_F32_TFLOPS, 4))
+                peak_tflops=peak, frac_issued=round(issued / dur / 1e12 / peak, 4))
 
 
 # HBM traffic per launch from FETCH_SIZE and WRITE_SIZE counter runs (scaled by the
@@ -567,6 +572,7 @@ def main():
                            stage_us=[round(x * 1e3, 2) for x in r["stage_ms"]],
                            stage2_GBs=roofline_for(r)["achieved"], finite=r["finite"],
                            waves_per_tile=r["tune"]["choice"],
+                           fp64_mfma=(mfma_stats(r, MFMA_PEAK_F64_TFLOPS) if r["tune"]["choice"] == 0 else None),
                            autotune_stage2_us={k: round(v * 1e3, 2) for k, v in r["tune"]["ms"].items()},
                            fp32_mfma=fp32.get(r["N"]))
                       for r in per],
