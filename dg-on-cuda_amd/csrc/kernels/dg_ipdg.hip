@@ -51,8 +51,18 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   const int c = c0 + lane;
   const bool valid = c < K;
   if (p.scal && p.scal[kDone] != T(0)) return;
+  // with dirz set the operand is the new search direction dirz + beta u
+  const T beta = p.dirz ? p.scal[kBeta] : T(0);
+  auto operand = [&](size_t idx) { return p.dirz ? T(fma(beta, p.u[idx], p.dirz[idx])) : p.u[idx]; };
 
-  for (int j = wave; j < Np; j += kWaves) us[j][lane] = valid ? p.u[(size_t)j * K + c] : T(0);
+  for (int j = wave; j < Np; j += kWaves) {
+    T v = 0;
+    if (valid) {
+      v = operand((size_t)j * K + c);
+      if (p.dirz) p.dirp[(size_t)j * K + c] = v;
+    }
+    us[j][lane] = v;
+  }
   __syncthreads();
 
   for (int m = wave; m < NF; m += kWaves) {
@@ -70,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
       } else {
         const int nn = Tr::fnode(nf, Nfp - 1 - d);
         const int l = idx - c0;
-        const T b = ((unsigned)l < (unsigned)kTile) ? us[nn][l] : p.u[(size_t)nn * K + idx];
+        const T b = ((unsigned)l < (unsigned)kTile) ? us[nn][l] : operand((size_t)nn * K + idx);
         du = a - b;
       }
       p.du[(size_t)m * K + c] = du;
@@ -304,9 +314,11 @@ __global__ __launch_bounds__(kBlock) void ipdg_mass(const IpdgParams<T> p) {
 }
 
 // CG update with block-Jacobi preconditioning, one tile of cells:
-//   x += alpha p;  r -= alpha q;  z = P_c r_c (P: per-cell Np x Np, entry
-//   (j,k) of cell c at P[(j*Np + k) * K + c]; identity if P is null);
-//   partials: r.z and r.r.
+//   x += alpha p;  r -= alpha q;  z = P_c r_c with P_c the cell's symmetric
+//   Np x Np inverse block stored packed (entry (j,k), j <= k, of cell c at
+//   P[packed_index(Np, j, k) * K + c]; the waves owning rows j and k read the
+//   shared entry close together, the second read usually from cache);
+//   identity if P is null; partials: r.z (or z.z) and r.r.
 template <int N, typename T>
 __global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
   using Tr = Tri<N>;
@@ -341,7 +353,10 @@ __global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
     if (p.P && valid) {
       z = 0;
 #pragma unroll
-      for (int k = 0; k < Np; ++k) z = fma(p.P[((size_t)j * Np + k) * K + c], rs[k][lane], z);
+      for (int k = 0; k < Np; ++k) {
+        const int t = k < j ? packed_index(Np, k, j) : packed_index(Np, j, k);
+        z = fma(p.P[(size_t)t * K + c], rs[k][lane], z);
+      }
     }
     if (valid) p.z[(size_t)j * K + c] = z;
     rz += (p.zz ? z : rs[j][lane]) * z;
@@ -404,14 +419,7 @@ __global__ void cg_scalars(const T* __restrict__ partial, int nparts, int which,
     scal[kPq] = T(sh[0][0]);
     scal[kAlpha] = T(double(scal[kRz]) / sh[0][0]);
   } else if (which == kAfterUpdate) {
-    const double rz = sh[0][0], rr = sh[1][0];
-    scal[kBeta] = T(rz / double(scal[kRz]));
-    scal[kRz] = T(rz);
-    scal[kRr] = T(rr);
-    const int it = int(scal[kIter]);
-    if (history) history[it + 1] = rr;  // history[0] holds the initial residual
-    scal[kIter] = T(it + 1);
-    if (rr <= double(scal[kTol2]) || it + 1 >= int(scal[kMaxit])) scal[kDone] = T(1);
+    cg_after_update(scal, history, sh[0][0], sh[1][0]);
   }
 }
 

@@ -34,6 +34,10 @@ struct IpdgParams {
   const T* dotv;  // optional: partial dot with out
   T* partial;     // [grid] partial sums (optional)
   const T* scal;  // optional CG scalars: the kernels return at once when scal[kDone] != 0
+  // CG search direction formed on the fly (optional): pass 1 applies the
+  // operator to u' = dirz + scal[kBeta] u and stores u' in dirp.
+  const T* dirz;
+  T* dirp;
 };
 
 template <typename T>
@@ -44,11 +48,27 @@ struct CgParams {
   const T* p;
   const T* q;
   T* z;
-  const T* P;     // block-Jacobi inverse blocks [Np*Np][K] or null
+  const T* P;     // block-Jacobi inverses, symmetric, packed: [Np(Np+1)/2][K] or null
   const T* scal;  // CG scalars
   T* partial;     // [2 * grid]: (r.z, r.r), or (z.z, r.r) when zz != 0
   int zz = 0;
 };
+
+// Index of entry (j, k), j <= k, of a symmetric n x n matrix stored as its
+// upper triangle row by row.
+__host__ __device__ constexpr int packed_index(int n, int j, int k) { return j * n - j * (j - 1) / 2 + (k - j); }
+
+// CG step kAfterUpdate from the summed (r.z, r.r) (thread 0 of the last workgroup).
+template <typename T>
+__device__ __forceinline__ void cg_after_update(T* scal, double* history, double rz, double rr) {
+  scal[kBeta] = T(rz / double(scal[kRz]));
+  scal[kRz] = T(rz);
+  scal[kRr] = T(rr);
+  const int it = int(scal[kIter]);
+  if (history) history[it + 1] = rr;  // history[0] holds the initial residual
+  scal[kIter] = T(it + 1);
+  if (rr <= double(scal[kTol2]) || it + 1 >= int(scal[kMaxit])) scal[kDone] = T(1);
+}
 
 template <typename T>
 hipError_t ipdg_apply(int order, const IpdgParams<T>& p, int mode, hipStream_t s);

@@ -122,8 +122,8 @@ int IpdgLevel<T>::init(const dgc_ipdg_s& s, const std::vector<double>& drT, cons
     own_geo = true;
   }
   if ((rc = alloc(&qx, n)) || (rc = alloc(&qy, n)) || (rc = alloc(&du, nf)) || (rc = alloc(&r, n)) ||
-      (rc = alloc(&z, n)) || (rc = alloc(&p, n)) || (rc = alloc(&q, n)) || (rc = alloc(&x, n)) ||
-      (rc = alloc(&partial, 2 * size_t(s.grid))) || (rc = alloc(&scal, dgc::kCgScalars)))
+      (rc = alloc(&z, n)) || (rc = alloc(&p, n)) || (rc = alloc(&p2, n)) || (rc = alloc(&q, n)) ||
+      (rc = alloc(&x, n)) || (rc = alloc(&partial, 2 * size_t(s.grid))) || (rc = alloc(&scal, dgc::kCgScalars)))
     return rc;
   return DGC_OK;
 }
@@ -145,6 +145,7 @@ void IpdgLevel<T>::release() {
   (void)hipFree(r);
   (void)hipFree(z);
   (void)hipFree(p);
+  (void)hipFree(p2);
   (void)hipFree(q);
   (void)hipFree(x);
   (void)hipFree(partial);
@@ -222,19 +223,26 @@ static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_t
     }
   }
   DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kInit, L.scal, history, st));
-  DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
+  // kInit sets beta = 0: the first iteration forms p = z + 0 p from the zeroed p
   return DGC_OK;
 }
 
-// One PCG iteration (5 kernels + the two passes of the operator).
+// One PCG iteration: pass 1 forms the search direction p' = z + beta p (into
+// the other direction buffer) and applies the first operator pass to it;
+// pass 2 gives q = S p' and the partials of p'.q; alpha; the update
+// (x, r, z = P r) [+ the coarse correction]; beta and the stopping test.
+// Iteration `it` of a solve reads direction buffer it % 2 and writes the other.
 template <typename T>
-static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
+static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int it) {
   hipStream_t st = s->disc->ctx->stream;
-  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  T* pold = (it & 1) ? L.p2 : L.p;
+  T* pnew = (it & 1) ? L.p : L.p2;
   dgc::IpdgParams<T> pr = L.params(*s);
-  pr.u = L.p;
+  pr.u = pold;
+  pr.dirz = L.z;
+  pr.dirp = pnew;
   pr.out = L.q;
-  pr.dotv = L.p;
+  pr.dotv = pnew;
   pr.partial = L.partial;
   pr.scal = L.scal;
   DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpd, st));
@@ -243,7 +251,7 @@ static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
   cp.K = s->disc->K;
   cp.x = L.x;
   cp.r = L.r;
-  cp.p = L.p;
+  cp.p = pnew;
   cp.q = L.q;
   cp.z = L.z;
   cp.P = L.use_P ? L.P : nullptr;
@@ -257,7 +265,6 @@ static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
     }
   }
   DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterUpdate, L.scal, history, st));
-  DGC_LAUNCH(dgc::cg_direction_launch<T>(L.z, L.p, L.scal, n, st));
   return DGC_OK;
 }
 
@@ -265,14 +272,14 @@ static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history) {
 template <typename T>
 static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, int* iters) {
   hipStream_t st = s->disc->ctx->stream;
-  const int chunk = 16;
+  const int chunk = 16;  // even: every chunk starts on direction buffer 0
   if (!L.graph || L.graph_hist != history || L.graph_P != L.use_P || L.graph_coarse != L.use_coarse) {
     if (L.graph) (void)hipGraphExecDestroy(L.graph);
     L.graph = nullptr;
     hipGraph_t g = nullptr;
     DGC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = DGC_OK;
-    for (int i = 0; i < chunk && rc == DGC_OK; ++i) rc = cg_iteration(s, L, history);
+    for (int i = 0; i < chunk && rc == DGC_OK; ++i) rc = cg_iteration(s, L, history, i);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != DGC_OK) {
       if (g) (void)hipGraphDestroy(g);
@@ -468,7 +475,7 @@ int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors) {
                          o, blocks);
       DGC_LAUNCH(hipGetLastError());
     }
-  std::vector<double> hb(size_t(Np) * Np * K), hinv(hb.size());
+  std::vector<double> hb(size_t(Np) * Np * K), hinv(size_t(Np) * (Np + 1) / 2 * K);
   hipError_t e = hipMemcpyAsync(hb.data(), blocks, hb.size() * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(blocks);
@@ -483,8 +490,10 @@ int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors) {
     } catch (const std::exception& ex) {
       return dgc_set_error(DGC_ERR_INVALID, std::string("block-Jacobi: cell block not invertible: ") + ex.what());
     }
+    // symmetric by construction: store the mean of (j,k) and (k,j), upper triangle
     for (int j = 0; j < Np; ++j)
-      for (int k = 0; k < Np; ++k) hinv[(size_t(j) * Np + k) * K + c] = Bi(j, k);
+      for (int k = j; k < Np; ++k)
+        hinv[size_t(dgc::packed_index(Np, j, k)) * K + c] = 0.5 * (Bi(j, k) + Bi(k, j));
   }
   (void)hipFree(s->f64.P);
   (void)hipFree(s->f32.P);

@@ -22,9 +22,10 @@ struct IpdgLevel {
   bool own_geo = false;
   T *qx = nullptr, *qy = nullptr, *du = nullptr;
   T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr;
+  T* p2 = nullptr;  // second search-direction buffer (p alternates between p and p2)
   T* partial = nullptr;
   T* scal = nullptr;
-  T* P = nullptr;  // block-Jacobi inverse blocks [Np*Np][K]
+  T* P = nullptr;  // block-Jacobi inverses, symmetric, packed [Np(Np+1)/2][K]
   bool use_P = false;
   bool use_coarse = false;  // add the coarse-space correction (FP64 level only)
   hipGraphExec_t graph = nullptr;

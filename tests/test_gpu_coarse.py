@@ -103,6 +103,59 @@ def _oracle_relres(mesh, disc, x_node, b_node):
     return np.linalg.norm(b - Sx) / np.linalg.norm(b)
 
 
+def test_two_level_pcg_iterates_follow_textbook_pcg(ctx):
+    """The fused two-level iteration (the prolongation's last workgroup
+    finishes beta and the stopping test) follows textbook PCG with
+    z = P_BJ r + Phi Ainv Phi^T r on the oracle's dense operator, Ainv the
+    device's FP32 inverse; two solves are bitwise identical."""
+    N, nside, g = 2, 8, 2
+    mesh = dgc_amd.Mesh.structured(nside, nside, -1, 1, -1, 1, jitter=0.1, seed=7)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    K, Np = disc.K, disc.Np
+    nc = op.coarse_space(mesh, g)
+    b = np.random.default_rng(6).standard_normal(K * Np)
+    bd = dgc_amd.DeviceArray.from_host(ctx, dgc_amd.cell_to_node_major(b, K, Np))
+    runs = []
+    for _ in range(2):
+        xd = dgc_amd.DeviceArray(ctx, K * Np)
+        res = op.pcg(bd, xd, rtol=1e-12, maxit=5000, precond=2, history=True)
+        runs.append((res, xd.download()))
+    assert runs[0][0]["iters"] == runs[1][0]["iters"]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    info = op.coarse_info()
+    Phi = _phi(info, K, Np, nc)
+    Ainv = info["Ainv"].astype(np.float64)
+    inv = [np.linalg.inv(B) for B in op.diag_blocks()]
+    inv = [0.5 * (A + A.T) for A in inv]
+
+    def apply(r):
+        zb = np.concatenate([inv[c] @ r[c * Np:(c + 1) * Np] for c in range(K)])
+        return zb + Phi @ (Ainv @ (Phi.T @ r))
+
+    S = _oracle_S(mesh, N)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = apply(r)
+    p = z.copy()
+    rz = r @ z
+    want = [np.sqrt(r @ r)]
+    for _ in range(20):
+        q = S @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = apply(r)
+        rz_new = r @ z
+        want.append(np.sqrt(r @ r))
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    got = runs[0][0]["history"][:21]
+    d = np.max(np.abs(got - np.array(want)) / np.array(want))
+    print(f"two-level: {runs[0][0]['iters']} it; max rel diff of the first 20 residual norms {d:.1e}")
+    assert d < 1e-8
+
+
 def test_two_level_pcg_converges_in_fewer_iterations(ctx):
     mesh, disc, op, b = _poisson(ctx, 32, 3)
     xb = dgc_amd.DeviceArray(ctx, disc.size)

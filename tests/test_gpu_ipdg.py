@@ -119,6 +119,63 @@ def test_pcg_small_matches_direct_solve(ctx, precond):
     assert np.all(np.isfinite(res["history"]))
 
 
+def _textbook_pcg(S, b, precond, iters):
+    """Residual norms of textbook PCG (x0 = 0) on a dense matrix."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = precond(r)
+    p = z.copy()
+    rz = r @ z
+    hist = [np.sqrt(r @ r)]
+    for _ in range(iters):
+        q = S @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = precond(r)
+        rz_new = r @ z
+        hist.append(np.sqrt(r @ r))
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return np.array(hist)
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_pcg_iterates_follow_textbook_pcg(ctx, precond):
+    """The fused iteration (the search direction formed inside the operator's
+    first pass; alpha, beta and the stopping test finished by the last
+    workgroup of each reduction) follows textbook PCG on the dense operator:
+    the residual norms of the first 25 iterations agree, and two solves are
+    bitwise identical."""
+    N = 2
+    mesh = dgc_amd.Mesh.structured(6, 6, -1, 1, -1, 1, jitter=0.1, seed=5)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    K, Np = disc.K, disc.Np
+    S = _dense_spd(Problem.from_mesh(mesh, N, form="strong"), mesh, N, 1)
+    b = np.random.default_rng(4).standard_normal(K * Np)
+    bd = dgc_amd.DeviceArray.from_host(ctx, c2n(b, K, Np))
+    runs = []
+    for _ in range(2):
+        xd = dgc_amd.DeviceArray(ctx, K * Np)
+        res = op.pcg(bd, xd, rtol=1e-12, maxit=5000, precond=precond, history=True)
+        runs.append((res, xd.download()))
+    assert runs[0][0]["iters"] == runs[1][0]["iters"]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    np.testing.assert_array_equal(runs[0][0]["history"], runs[1][0]["history"])
+    if precond == 1:
+        inv = [np.linalg.inv(B) for B in op.diag_blocks()]
+        inv = [0.5 * (A + A.T) for A in inv]
+        apply = lambda r: np.concatenate([inv[c] @ r[c * Np:(c + 1) * Np] for c in range(K)])
+    else:
+        apply = lambda r: r.copy()
+    want = _textbook_pcg(S, b, apply, 25)
+    got = runs[0][0]["history"][:26]
+    d = np.max(np.abs(got - want) / want)
+    print(f"precond {precond}: {runs[0][0]['iters']} it; max rel diff of the first 25 residual norms {d:.1e}")
+    assert d < 1e-8
+
+
 def _poisson_problem(ctx, n_side, N, seed=3):
     mesh = dgc_amd.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=seed)
     disc = dgc_amd.Discretization(ctx, mesh, N)
