@@ -80,26 +80,35 @@ __global__ __launch_bounds__(256) void restrict_kernel(const CoarseSpace cs, con
   }
 }
 
-// yc = Ainv rc: rc staged in LDS once per workgroup, one wave per row,
-// 16-byte row loads, FP64 accumulation.
+// yc = Ainv rc: one wave per row, 16-byte row loads with rc read through the
+// cache next to them (rc holds ld entries, the padding zero), four
+// independent partial sums per lane, FP64 accumulation in a fixed order.
 __global__ __launch_bounds__(256) void solve_kernel(const CoarseSpace cs, const double* scal) {
-  extern __shared__ double rcs[];
   if (finished(scal)) return;
-  const int nc = cs.nc, ld = cs.ld;
-  for (int i = threadIdx.x; i < ld; i += 256) rcs[i] = i < nc ? cs.rc[i] : 0.0;
-  __syncthreads();
+  const int nc = cs.nc, ld4 = cs.ld / 4;
   const int lane = threadIdx.x & 63;
-  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nc; row += gridDim.x * 4) {
-    const float4* a = reinterpret_cast<const float4*>(cs.Ainv + (size_t)row * ld);
-    double acc = 0.0;
-    for (int q = lane; q < ld / 4; q += 64) {
-      const float4 w = a[q];
-      acc += double(w.x) * rcs[4 * q] + double(w.y) * rcs[4 * q + 1] + double(w.z) * rcs[4 * q + 2] +
-             double(w.w) * rcs[4 * q + 3];
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= nc) return;
+  const float4* a = reinterpret_cast<const float4*>(cs.Ainv + (size_t)row * cs.ld);
+  const double2* x = reinterpret_cast<const double2*>(cs.rc);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int q = lane;
+  for (; q + 192 < ld4; q += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = q + 64 * u;
+      const float4 w = a[k];
+      const double2 x0 = x[2 * k], x1 = x[2 * k + 1];
+      acc[u] += double(w.x) * x0.x + double(w.y) * x0.y + double(w.z) * x1.x + double(w.w) * x1.y;
     }
-    acc = wave_sum(acc);
-    if (lane == 0) cs.yc[row] = acc;
   }
+  for (; q < ld4; q += 64) {
+    const float4 w = a[q];
+    const double2 x0 = x[2 * q], x1 = x[2 * q + 1];
+    acc[0] += double(w.x) * x0.x + double(w.y) * x0.y + double(w.z) * x1.x + double(w.w) * x1.y;
+  }
+  const double sum = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  if (lane == 0) cs.yc[row] = sum;
 }
 
 // z += Phi yc over a tile of 64 cells (lane = cell, waves split the nodes);
@@ -218,8 +227,8 @@ hipError_t coarse_restrict(int order, const CoarseSpace& cs, const double* v, do
 }
 
 hipError_t coarse_solve(const CoarseSpace& cs, const double* scal, hipStream_t s) {
-  const int grid = std::min(256, (cs.nc + 3) / 4);
-  hipLaunchKernelGGL(solve_kernel, dim3(grid), dim3(256), size_t(cs.ld) * sizeof(double), s, cs, scal);
+  const int grid = (cs.nc + 3) / 4;
+  hipLaunchKernelGGL(solve_kernel, dim3(grid), dim3(256), 0, s, cs, scal);
   return hipGetLastError();
 }
 

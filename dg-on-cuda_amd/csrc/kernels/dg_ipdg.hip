@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp;
   constexpr int RPW = (Np + kWaves - 1) / kWaves;
   constexpr int Npad = RPW * kWaves;
+  constexpr int FPW = (NF + kWaves - 1) / kWaves;  // face slots per wave (m = wave + 4 i)
   __shared__ T us[Np][kTile];
   __shared__ T lx[NF][kTile];
   __shared__ T ly[NF][kTile];
@@ -50,43 +51,112 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   const int c0 = blockIdx.x * kTile;
   const int c = c0 + lane;
   const bool valid = c < K;
-  if (p.scal && p.scal[kDone] != T(0)) return;
-  // with dirz set the operand is the new search direction dirz + beta u
-  const T beta = p.dirz ? p.scal[kBeta] : T(0);
-  auto operand = [&](size_t idx) { return p.dirz ? T(fma(beta, p.u[idx], p.dirz[idx])) : p.u[idx]; };
-
-  for (int j = wave; j < Np; j += kWaves) {
-    T v = 0;
-    if (valid) {
-      v = operand((size_t)j * K + c);
-      if (p.dirz) p.dirp[(size_t)j * K + c] = v;
+  const bool fused = p.rec_out != nullptr;
+  if (fused) {
+    if (p.scal[kDone] != T(0)) {  // finished: carry the record over to the other slot
+      if (blockIdx.x == 0 && threadIdx.x < kCgScalars) p.rec_out[threadIdx.x] = p.scal[threadIdx.x];
+      return;
     }
-    us[j][lane] = v;
+  } else if (p.scal && p.scal[kDone] != T(0)) {
+    return;
+  }
+  // With dirz set the operand is the new search direction dirz + beta u.
+  auto raw_z = [&](size_t idx) { return p.dirz ? p.dirz[idx] : T(0); };
+
+  // ---- every independent read first: node rows, connectivity, face and
+  // metric terms; then the neighbour traces outside the tile
+  T own[RPW], ownz[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = wave + kWaves * i;
+    const bool ok = valid && j < Np;
+    own[i] = ok ? p.u[(size_t)j * K + c] : T(0);
+    ownz[i] = ok ? raw_z((size_t)j * K + c) : T(0);
+  }
+  int code[FPW];
+  T fnx[FPW], fny[FPW], ffs[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    const int e = m / Nfp;
+    code[i] = kBoundaryCode;
+    fnx[i] = fny[i] = ffs[i] = 0;
+    if (valid && m < NF) {
+      code[i] = p.nbr[(size_t)e * K + c];
+      fnx[i] = p.geo[(size_t)(kNx0 + e) * K + c];
+      fny[i] = p.geo[(size_t)(kNy0 + e) * K + c];
+      ffs[i] = p.geo[(size_t)(kFs0 + e) * K + c];
+    }
+  }
+  T rx = 0, ry = 0, sx = 0, sy = 0;
+  if (valid) {
+    rx = p.geo[(size_t)kRx * K + c];
+    ry = p.geo[(size_t)kRy * K + c];
+    sx = p.geo[(size_t)kSx * K + c];
+    sy = p.geo[(size_t)kSy * K + c];
+  }
+  T far[FPW], farz[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    const int d = m % Nfp;
+    const int nf = code[i] & 3, idx = code[i] >> 2;
+    far[i] = farz[i] = 0;
+    if (nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile) {
+      const size_t g = (size_t)Tr::fnode(nf, Nfp - 1 - d) * K + idx;
+      far[i] = p.u[g];
+      farz[i] = raw_z(g);
+    }
+  }
+  // ---- beta: from the record, or (fused) from the update's partials, with
+  // the stopping test and the next record
+  T beta = 0;
+  if (fused) {
+    double sums[2];
+    sum_partials<2>(p.upart, p.nparts, sums);
+    const CgAdvance adv = cg_advance(p.scal, sums[0], sums[1], p.scal[kPhase] != T(0));
+    if (blockIdx.x == 0 && threadIdx.x == 0) cg_store(p.scal, p.rec_out, p.history, adv, sums[0], sums[1]);
+    if (adv.done) return;
+    beta = T(adv.beta);
+  } else if (p.dirz) {
+    beta = p.scal[kBeta];
+  }
+  if (p.dirz) {
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) own[i] = fma(beta, own[i], ownz[i]);
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) far[i] = fma(beta, far[i], farz[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = wave + kWaves * i;
+    if (j < Np) {
+      us[j][lane] = own[i];
+      if (p.dirz && valid) p.dirp[(size_t)j * K + c] = own[i];
+    }
   }
   __syncthreads();
 
-  for (int m = wave; m < NF; m += kWaves) {
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const T a = us[Tr::fnode(e, d)][lane];
-    T du = 0, nx = 0, ny = 0, fs = 0;
+    T du = 0;
     if (valid) {
-      nx = p.geo[(size_t)(kNx0 + e) * K + c];
-      ny = p.geo[(size_t)(kNy0 + e) * K + c];
-      fs = p.geo[(size_t)(kFs0 + e) * K + c];
-      const int code = p.nbr[(size_t)e * K + c];
-      const int nf = code & 3, idx = code >> 2;
+      const int nf = code[i] & 3, idx = code[i] >> 2;
       if (nf == kBoundaryCode) {
         du = T(2) * a;
       } else {
-        const int nn = Tr::fnode(nf, Nfp - 1 - d);
         const int l = idx - c0;
-        const T b = ((unsigned)l < (unsigned)kTile) ? us[nn][l] : operand((size_t)nn * K + idx);
+        const T b = ((unsigned)l < (unsigned)kTile) ? us[Tr::fnode(nf, Nfp - 1 - d)][l] : far[i];
         du = a - b;
       }
       p.du[(size_t)m * K + c] = du;
     }
-    lx[m][lane] = nx * du * fs / T(2);
-    ly[m][lane] = ny * du * fs / T(2);
+    lx[m][lane] = fnx[i] * du * ffs[i] / T(2);
+    ly[m][lane] = fny[i] * du * ffs[i] / T(2);
   }
   __syncthreads();
 
@@ -104,13 +174,6 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
       dr[i] = fma(a[i], uk, dr[i]);
       ds[i] = fma(b[i], uk, ds[i]);
     }
-  }
-  T rx = 0, ry = 0, sx = 0, sy = 0;
-  if (valid) {
-    rx = p.geo[(size_t)kRx * K + c];
-    ry = p.geo[(size_t)kRy * K + c];
-    sx = p.geo[(size_t)kSx * K + c];
-    sy = p.geo[(size_t)kSy * K + c];
   }
   T qx[RPW], qy[RPW];
 #pragma unroll
@@ -157,25 +220,82 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
   const int c = c0 + lane;
   const bool valid = c < K;
   if (p.scal && p.scal[kDone] != T(0)) return;
+  constexpr int FPW = (NF + kWaves - 1) / kWaves;  // face slots per wave (m = wave + 4 i)
 
-  for (int j = wave; j < Np; j += kWaves) {
-    qxs[j][lane] = valid ? p.qx[(size_t)j * K + c] : T(0);
-    qys[j][lane] = valid ? p.qy[(size_t)j * K + c] : T(0);
+  // ---- every independent read first: q rows, connectivity, face terms,
+  // penalties, jumps and metric terms; then the neighbours' q outside the tile
+  T ownx[RPW], owny[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = wave + kWaves * i;
+    const bool ok = valid && j < Np;
+    ownx[i] = ok ? p.qx[(size_t)j * K + c] : T(0);
+    owny[i] = ok ? p.qy[(size_t)j * K + c] : T(0);
+  }
+  int code[FPW];
+  T fnx[FPW], fny[FPW], ffs[FPW], ftau[FPW], fdu[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    const int e = m / Nfp;
+    code[i] = kBoundaryCode;
+    fnx[i] = fny[i] = ffs[i] = ftau[i] = fdu[i] = 0;
+    if (valid && m < NF) {
+      code[i] = p.nbr[(size_t)e * K + c];
+      fnx[i] = p.geo[(size_t)(kNx0 + e) * K + c];
+      fny[i] = p.geo[(size_t)(kNy0 + e) * K + c];
+      ffs[i] = p.geo[(size_t)(kFs0 + e) * K + c];
+      ftau[i] = p.tau[(size_t)e * K + c];
+      fdu[i] = p.du[(size_t)m * K + c];
+    }
+  }
+  T rx = 0, ry = 0, sx = 0, sy = 0;
+  if (valid) {
+    rx = p.geo[(size_t)kRx * K + c];
+    ry = p.geo[(size_t)kRy * K + c];
+    sx = p.geo[(size_t)kSx * K + c];
+    sy = p.geo[(size_t)kSy * K + c];
+  }
+  // SPD form: J and the dot operand at this wave's output rows j0 + i
+  const T Jc = (MODE == kSpd && valid) ? p.J[c] : T(0);
+  T dv[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = wave * RPW + i;
+    dv[i] = (MODE == kSpd && p.dotv && valid && j < Np) ? p.dotv[(size_t)j * K + c] : T(0);
+  }
+  T farx[FPW], fary[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    const int d = m % Nfp;
+    const int nf = code[i] & 3, idx = code[i] >> 2;
+    farx[i] = fary[i] = 0;
+    if (nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile) {
+      const size_t g = (size_t)Tr::fnode(nf, Nfp - 1 - d) * K + idx;
+      farx[i] = p.qx[g];
+      fary[i] = p.qy[g];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int j = wave + kWaves * i;
+    if (j < Np) {
+      qxs[j][lane] = ownx[i];
+      qys[j][lane] = owny[i];
+    }
   }
   __syncthreads();
 
-  for (int m = wave; m < NF; m += kWaves) {
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int m = wave + kWaves * i;
+    if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const int own = Tr::fnode(e, d);
     T v = 0;
     if (valid) {
-      const T nx = p.geo[(size_t)(kNx0 + e) * K + c];
-      const T ny = p.geo[(size_t)(kNy0 + e) * K + c];
-      const T fs = p.geo[(size_t)(kFs0 + e) * K + c];
-      const T tau = p.tau[(size_t)e * K + c];
-      const T du = p.du[(size_t)m * K + c];
-      const int code = p.nbr[(size_t)e * K + c];
-      const int nf = code & 3, idx = code >> 2;
+      const int nf = code[i] & 3, idx = code[i] >> 2;
       T dqn = 0;
       if (nf != kBoundaryCode) {
         const int nn = Tr::fnode(nf, Nfp - 1 - d);
@@ -185,12 +305,12 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
           bx = qxs[nn][l];
           by = qys[nn][l];
         } else {
-          bx = p.qx[(size_t)nn * K + idx];
-          by = p.qy[(size_t)nn * K + idx];
+          bx = farx[i];
+          by = fary[i];
         }
-        dqn = (qxs[own][lane] - bx) * nx + (qys[own][lane] - by) * ny;
+        dqn = (qxs[own][lane] - bx) * fnx[i] + (qys[own][lane] - by) * fny[i];
       }
-      v = (dqn + T(2) * (tau * du)) * fs / T(2);
+      v = (dqn + T(2) * (ftau[i] * fdu[i])) * ffs[i] / T(2);
     }
     lf[m][lane] = v;
   }
@@ -212,13 +332,6 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
       ay[i] = fma(a[i], rk, ay[i]);
       by[i] = fma(b[i], rk, by[i]);
     }
-  }
-  T rx = 0, ry = 0, sx = 0, sy = 0;
-  if (valid) {
-    rx = p.geo[(size_t)kRx * K + c];
-    ry = p.geo[(size_t)kRy * K + c];
-    sx = p.geo[(size_t)kSx * K + c];
-    sy = p.geo[(size_t)kSy * K + c];
   }
   T lap[RPW];
 #pragma unroll
@@ -249,7 +362,6 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
     if (j < Np) qxs[j][lane] = lap[i];
   }
   __syncthreads();
-  const T Jc = valid ? p.J[c] : T(0);
   T o[RPW];
 #pragma unroll
   for (int i = 0; i < RPW; ++i) o[i] = 0;
@@ -268,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
       if (j < Np) {
         const T val = -Jc * o[i];
         p.out[(size_t)j * K + c] = val;
-        if (p.dotv) dot += p.dotv[(size_t)j * K + c] * val;
+        dot += dv[i] * val;
       }
     }
   }
@@ -323,6 +435,7 @@ template <int N, typename T>
 __global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
   using Tr = Tri<N>;
   constexpr int Np = Tr::Np;
+  constexpr int LPW = (Np + kWaves - 1) / kWaves;  // rows per wave (j = wave + 4 i)
   __shared__ T rs[Np][kTile];
   __shared__ T red[kWaves];
   const int K = p.K;
@@ -330,19 +443,42 @@ __global__ __launch_bounds__(kBlock) void cg_update(const CgParams<T> p) {
   const int wave = uniform(threadIdx.x >> 6);
   const int c = blockIdx.x * kTile + lane;
   const bool valid = c < K;
-  const T alpha = p.scal[kAlpha];
-  const bool skip = p.scal[kDone] != T(0);
+  const bool skip = p.scal[kDone] != T(0);  // initialisation: only z and the partials
+  if (skip && p.dpart) return;              // fused iteration after the solve finished
+  T rv[LPW], xv[LPW], pv[LPW], qv[LPW];
+#pragma unroll
+  for (int i = 0; i < LPW; ++i) {
+    const int j = wave + kWaves * i;
+    const bool ok = valid && j < Np;
+    const size_t idx = (size_t)j * K + c;
+    rv[i] = ok ? p.r[idx] : T(0);
+    xv[i] = (ok && !skip) ? p.x[idx] : T(0);
+    pv[i] = (ok && !skip) ? p.p[idx] : T(0);
+    qv[i] = (ok && !skip) ? p.q[idx] : T(0);
+  }
+  T alpha;
+  if (p.dpart) {
+    double pq[1];
+    sum_partials<1>(p.dpart, p.nparts, pq);
+    alpha = T(double(p.scal[kRz]) / pq[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      p.rec_out[kPq] = T(pq[0]);
+      p.rec_out[kAlpha] = alpha;
+    }
+  } else {
+    alpha = p.scal[kAlpha];
+  }
   T rr = 0, rz = 0;
-  for (int j = wave; j < Np; j += kWaves) {
-    T r = 0;
-    if (valid) {
+#pragma unroll
+  for (int i = 0; i < LPW; ++i) {
+    const int j = wave + kWaves * i;
+    if (j >= Np) break;
+    T r = rv[i];
+    if (valid && !skip) {
       const size_t idx = (size_t)j * K + c;
-      r = p.r[idx];
-      if (!skip) {
-        p.x[idx] = p.x[idx] + alpha * p.p[idx];
-        r = r - alpha * p.q[idx];
-        p.r[idx] = r;
-      }
+      p.x[idx] = xv[i] + alpha * pv[i];
+      r = r - alpha * qv[i];
+      p.r[idx] = r;
     }
     rs[j][lane] = r;
     rr += r * r;
@@ -380,47 +516,29 @@ __global__ void cg_direction(const T* __restrict__ z, T* __restrict__ pv, const 
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) pv[i] = z[i] + beta * pv[i];
 }
 
-// Finish a reduction of `nparts` partials (stride `stride`, offset `off`),
-// then update the CG scalars:
-//   which == kAfterApply:  alpha = rz / (p.q)
+// Finish a reduction of `nparts` partials (the same fixed order as the fused
+// iteration's kernels), then update the CG scalars:
+//   which == kInit:        rz = r.z, rr = r.r, beta = 0, history[0]; done if rr <= tol2
+//   which == kAfterApply:  pq = p.q, alpha = rz / pq
 //   which == kAfterUpdate: beta = rz_new / rz, rz = rz_new, rr = r.r,
-//                          history[it] = rr, done if rr <= tol2 or it+1 >= maxit
+//                          history[it] = rr, done if rr <= tol2 or it >= maxit
 template <typename T>
-__global__ void cg_scalars(const T* __restrict__ partial, int nparts, int which, T* scal,
-                           double* history) {
-  __shared__ double sh[2][1024];
-  double a = 0.0, b = 0.0;
-  const int width = which == kAfterApply ? 1 : 2;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    a += double(partial[width * i]);
-    if (width == 2) b += double(partial[width * i + 1]);
-  }
-  sh[0][threadIdx.x] = a;
-  sh[1][threadIdx.x] = b;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  if (which == kInit) {  // rz = r.z, rr = r.r, beta = 0; clears the done flag
-    scal[kRz] = T(sh[0][0]);
-    scal[kRr] = T(sh[1][0]);
-    scal[kBeta] = T(0);
-    if (history) history[0] = sh[1][0];
-    scal[kDone] = T(sh[1][0] <= double(scal[kTol2]) ? 1 : 0);
+__global__ __launch_bounds__(kBlock) void cg_scalars(const T* __restrict__ partial, int nparts, int which, T* scal,
+                                                     double* history) {
+  if (which == kAfterApply) {
+    double pq[1];
+    sum_partials<1>(partial, nparts, pq);
+    if (threadIdx.x != 0 || scal[kDone] != T(0)) return;
+    scal[kPq] = T(pq[0]);
+    scal[kAlpha] = T(double(scal[kRz]) / pq[0]);
     return;
   }
-  if (scal[kDone] != T(0)) return;
-  if (which == kAfterApply) {
-    scal[kPq] = T(sh[0][0]);
-    scal[kAlpha] = T(double(scal[kRz]) / sh[0][0]);
-  } else if (which == kAfterUpdate) {
-    cg_after_update(scal, history, sh[0][0], sh[1][0]);
-  }
+  double sums[2];
+  sum_partials<2>(partial, nparts, sums);
+  if (threadIdx.x != 0) return;
+  if (which == kAfterUpdate && scal[kDone] != T(0)) return;
+  const CgAdvance adv = cg_advance(scal, sums[0], sums[1], which == kInit);
+  cg_store(scal, scal, history, adv, sums[0], sums[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,10 +547,11 @@ namespace {
 template <typename T, int N>
 hipError_t launch_apply_n(const IpdgParams<T>& p, int mode, hipStream_t s) {
   const int grid = (p.K + kTile - 1) / kTile;
-  if (mode != kPhaseTwo) hipLaunchKernelGGL((ipdg_pass1<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
+  if (mode != kPhaseTwo && mode != kSpdPhaseTwo)
+    hipLaunchKernelGGL((ipdg_pass1<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
   if (mode == kLaplace || mode == kPhaseTwo)
     hipLaunchKernelGGL((ipdg_pass2<N, T, kLaplace>), dim3(grid), dim3(kBlock), 0, s, p);
-  else if (mode == kSpd)
+  else if (mode == kSpd || mode == kSpdPhaseTwo)
     hipLaunchKernelGGL((ipdg_pass2<N, T, kSpd>), dim3(grid), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }
@@ -496,7 +615,7 @@ hipError_t cg_direction_launch(const T* z, T* pv, const T* scal, size_t n, hipSt
 template <typename T>
 hipError_t cg_scalars_launch(const T* partial, int nparts, int which, T* scal, double* history,
                              hipStream_t s) {
-  hipLaunchKernelGGL((cg_scalars<T>), dim3(1), dim3(1024), 0, s, partial, nparts, which, scal, history);
+  hipLaunchKernelGGL((cg_scalars<T>), dim3(1), dim3(kBlock), 0, s, partial, nparts, which, scal, history);
   return hipGetLastError();
 }
 

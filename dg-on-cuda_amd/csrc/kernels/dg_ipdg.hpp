@@ -5,14 +5,24 @@
 namespace dgc {
 
 // kLaplace: both passes, out = Lap u; kSpd: both passes, out = -J M Lap u;
-// kPhaseOne / kPhaseTwo: one pass only (the reference's two-phase launch).
-enum IpdgMode { kLaplace = 0, kSpd = 1, kPhaseOne = 2, kPhaseTwo = 3 };
+// kPhaseOne / kPhaseTwo: one pass only (the reference's two-phase launch);
+// kSpdPhaseTwo: the second pass of kSpd alone.
+enum IpdgMode { kLaplace = 0, kSpd = 1, kPhaseOne = 2, kPhaseTwo = 3, kSpdPhaseTwo = 4 };
 // CG scalar slots (device array of 8 values of the solver's type)
 enum CgScalar {
   kAlpha = 0, kBeta = 1, kRz = 2, kRr = 3, kIter = 4, kDone = 5, kTol2 = 6, kMaxit = 7,
   kPq = 8,  // p.q of the current iteration (flexible CG's beta)
+  kPhase = 9,  // fused iteration: 1 while the record holds a fresh initialisation
   kCgScalars = 10
 };
+// The fused PCG iteration keeps two records of CG scalars (stride
+// kRecStride), the one an iteration reads and the one it writes, alternating
+// with the iteration's parity, so no kernel writes a value that other
+// workgroups of the same kernel still read.
+constexpr int kRecStride = 16;
+// Largest partial-sum count for which the fused iteration's kernels finish
+// the reductions themselves (every workgroup sums all partials, from cache).
+constexpr int kFusedMaxParts = 4096;
 enum CgStep { kInit = 0, kAfterApply = 1, kAfterUpdate = 2 };
 
 template <typename T>
@@ -35,9 +45,17 @@ struct IpdgParams {
   T* partial;     // [grid] partial sums (optional)
   const T* scal;  // optional CG scalars: the kernels return at once when scal[kDone] != 0
   // CG search direction formed on the fly (optional): pass 1 applies the
-  // operator to u' = dirz + scal[kBeta] u and stores u' in dirp.
+  // operator to u' = dirz + beta u and stores u' in dirp, beta = scal[kBeta].
   const T* dirz;
   T* dirp;
+  // Fused iteration (optional, pass 1): scal is the record this iteration
+  // reads; the kernel sums the update's partials upart[2 i], upart[2 i + 1]
+  // (r.z, r.r; i < nparts) itself, takes beta and the stopping test from
+  // them, and workgroup 0 writes the next record rec_out and history.
+  const T* upart;
+  int nparts;
+  T* rec_out;
+  double* history;
 };
 
 template <typename T>
@@ -52,22 +70,77 @@ struct CgParams {
   const T* scal;  // CG scalars
   T* partial;     // [2 * grid]: (r.z, r.r), or (z.z, r.r) when zz != 0
   int zz = 0;
+  // Fused iteration (optional): the kernel sums pass 2's partials
+  // dpart[i] (p.q, i < nparts) itself for alpha = scal[kRz] / p.q, and
+  // workgroup 0 stores alpha and p.q in rec_out (the record scal points to).
+  const T* dpart = nullptr;
+  int nparts = 0;
+  T* rec_out = nullptr;
 };
+
+// Fixed-order sum of W interleaved series partial[W i + w], i < nparts, in
+// double, over a 256-thread workgroup; every thread receives the sums. Each
+// thread adds a strided subset in index order, waves combine by butterfly
+// shuffles (identical in every lane: IEEE addition commutes), wave totals
+// are added in wave order. Every workgroup, and the one-workgroup scalar
+// kernel, gets bitwise the same sums.
+template <int W, typename T>
+__device__ __forceinline__ void sum_partials(const T* partial, int nparts, double (&out)[W]) {
+  __shared__ double wsum[W][kWaves];
+  double a[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) a[w] = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock)
+#pragma unroll
+    for (int w = 0; w < W; ++w) a[w] += double(partial[W * i + w]);
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    for (int off = 32; off > 0; off >>= 1) a[w] += __shfl_xor(a[w], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int w = 0; w < W; ++w) wsum[w][threadIdx.x >> 6] = a[w];
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < W; ++w) out[w] = ((wsum[w][0] + wsum[w][1]) + wsum[w][2]) + wsum[w][3];
+  __syncthreads();
+}
 
 // Index of entry (j, k), j <= k, of a symmetric n x n matrix stored as its
 // upper triangle row by row.
 __host__ __device__ constexpr int packed_index(int n, int j, int k) { return j * n - j * (j - 1) / 2 + (k - j); }
 
-// CG step kAfterUpdate from the summed (r.z, r.r) (thread 0 of the last workgroup).
+// CG scalars after a residual update from the summed (r.z, r.r); after the
+// initialisation (init) beta = 0 and the iteration count stays 0.
+struct CgAdvance {
+  double beta;
+  int it;
+  bool done;
+};
 template <typename T>
-__device__ __forceinline__ void cg_after_update(T* scal, double* history, double rz, double rr) {
-  scal[kBeta] = T(rz / double(scal[kRz]));
-  scal[kRz] = T(rz);
-  scal[kRr] = T(rr);
-  const int it = int(scal[kIter]);
-  if (history) history[it + 1] = rr;  // history[0] holds the initial residual
-  scal[kIter] = T(it + 1);
-  if (rr <= double(scal[kTol2]) || it + 1 >= int(scal[kMaxit])) scal[kDone] = T(1);
+__device__ __forceinline__ CgAdvance cg_advance(const T* in, double rz, double rr, bool init) {
+  CgAdvance s;
+  s.beta = init ? 0.0 : rz / double(in[kRz]);
+  s.it = init ? 0 : int(in[kIter]) + 1;
+  s.done = rr <= double(in[kTol2]) || (!init && s.it >= int(in[kMaxit]));
+  return s;
+}
+// Store the record after advance s (in and out may be the same record);
+// history[it] = r.r (history[0]: the initial residual).
+template <typename T>
+__device__ __forceinline__ void cg_store(const T* in, T* out, double* history, const CgAdvance& s, double rz,
+                                         double rr) {
+  const T alpha = in[kAlpha], tol2 = in[kTol2], maxit = in[kMaxit], pq = in[kPq];
+  out[kAlpha] = alpha;
+  out[kBeta] = T(s.beta);
+  out[kRz] = T(rz);
+  out[kRr] = T(rr);
+  out[kIter] = T(s.it);
+  out[kDone] = T(s.done ? 1 : 0);
+  out[kTol2] = tol2;
+  out[kMaxit] = maxit;
+  out[kPq] = pq;
+  out[kPhase] = T(0);
+  if (history) history[s.it] = rr;
 }
 
 template <typename T>
@@ -95,7 +168,7 @@ struct CoarseSpace {
   const float* coef = nullptr;        // [6][K]
   const double* rs = nullptr;         // [2][Np] reference node coordinates (rho, sigma)
   const float* Ainv = nullptr;        // [nc][ld] inverse of Phi^T S Phi (fp32, symmetric)
-  double* rc = nullptr;               // [nc] restricted vector
+  double* rc = nullptr;               // [ld] restricted vector (entries nc..ld-1 stay zero)
   double* yc = nullptr;               // [nc] coarse solution
 };
 // out[(3a + k) * ostride] = sum over the aggregate's nodes of phi_k v (wave per aggregate)

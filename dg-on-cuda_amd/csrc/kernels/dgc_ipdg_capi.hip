@@ -12,6 +12,7 @@
 #include "../host/refelem.hpp"
 #include "dg_internal.hpp"
 #include "dg_ipdg.hpp"
+#include "dg_mpnn.hpp"
 #include "dgc_ipdg_internal.hpp"
 
 #define DGC_HIP_TRY(call)                                                                     \
@@ -123,7 +124,8 @@ int IpdgLevel<T>::init(const dgc_ipdg_s& s, const std::vector<double>& drT, cons
   }
   if ((rc = alloc(&qx, n)) || (rc = alloc(&qy, n)) || (rc = alloc(&du, nf)) || (rc = alloc(&r, n)) ||
       (rc = alloc(&z, n)) || (rc = alloc(&p, n)) || (rc = alloc(&p2, n)) || (rc = alloc(&q, n)) ||
-      (rc = alloc(&x, n)) || (rc = alloc(&partial, 2 * size_t(s.grid))) || (rc = alloc(&scal, dgc::kCgScalars)))
+      (rc = alloc(&x, n)) || (rc = alloc(&partial, 2 * size_t(s.grid))) || (rc = alloc(&dpartial, size_t(s.grid))) ||
+      (rc = alloc(&scal, dgc::kCgScalars)) || (rc = alloc(&rec, 2 * dgc::kRecStride)))
     return rc;
   return DGC_OK;
 }
@@ -149,7 +151,9 @@ void IpdgLevel<T>::release() {
   (void)hipFree(q);
   (void)hipFree(x);
   (void)hipFree(partial);
+  (void)hipFree(dpartial);
   (void)hipFree(scal);
+  (void)hipFree(rec);
   (void)hipFree(P);
   if (graph) (void)hipGraphExecDestroy(graph);
   *this = IpdgLevel<T>();
@@ -176,16 +180,37 @@ dgc::IpdgParams<T> IpdgLevel<T>::params(const dgc_ipdg_s& s) const {
 template struct IpdgLevel<double>;
 template struct IpdgLevel<float>;
 
-// Enqueue CG initialisation: r = b - S x (x given), z = P r, p = z, scalars.
+// The fused iteration (3 kernels per block-Jacobi iteration: the scalar
+// reductions are finished inside the kernels that consume them) runs when
+// the partial-sum count is small enough for every workgroup to sum them all.
+static bool fused_iteration(const dgc_ipdg_s* s) { return s->grid <= dgc::kFusedMaxParts; }
+
+// Enqueue CG initialisation: r = b - S x (x given), z = P r and the partials
+// (r.z, r.r); p = 0 with beta = 0, so the first iteration's direction is z.
+// Unfused: the scalar kernel finishes the initial scalars. Fused: record 0
+// marks a fresh initialisation (kPhase) for the first iteration's pass 1, and
+// record 1 carries the done flag that makes the update form z only.
 template <typename T>
 static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_times_bb, int maxit,
                    bool x_is_zero, double* history) {
   hipStream_t st = s->disc->ctx->stream;
   const size_t n = size_t(s->disc->K) * s->disc->Np;
-  // done = 1 while initialising: cg_update then leaves x and r alone and only
-  // forms z = P r and the partial sums; the kInit reduction clears the flag.
-  T h[dgc::kCgScalars] = {0, 0, 0, 0, 0, 1, T(tol_rel2_times_bb), T(maxit)};
-  DGC_HIP_TRY(hipMemcpyAsync(L.scal, h, sizeof(h), hipMemcpyHostToDevice, st));
+  const bool fused = fused_iteration(s);
+  T* init_scal = L.scal;
+  if (fused) {
+    T h[2 * dgc::kRecStride] = {};
+    h[dgc::kTol2] = T(tol_rel2_times_bb);
+    h[dgc::kMaxit] = T(maxit);
+    h[dgc::kPhase] = T(1);
+    h[dgc::kRecStride + dgc::kDone] = T(1);
+    DGC_HIP_TRY(hipMemcpyAsync(L.rec, h, sizeof(h), hipMemcpyHostToDevice, st));
+    init_scal = L.rec + dgc::kRecStride;
+  } else {
+    // done = 1 while initialising: cg_update then leaves x and r alone and
+    // only forms z = P r and the partial sums; the kInit reduction clears it
+    T h[dgc::kCgScalars] = {0, 0, 0, 0, 0, 1, T(tol_rel2_times_bb), T(maxit)};
+    DGC_HIP_TRY(hipMemcpyAsync(L.scal, h, sizeof(h), hipMemcpyHostToDevice, st));
+  }
   if (x_is_zero) {
     DGC_HIP_TRY(hipMemcpyAsync(L.r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st));
   } else {
@@ -212,7 +237,7 @@ static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_t
   cp.q = L.q;
   cp.z = L.z;
   cp.P = L.use_P ? L.P : nullptr;
-  cp.scal = L.scal;
+  cp.scal = init_scal;
   cp.partial = L.partial;
   DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
   if constexpr (std::is_same<T, double>::value) {
@@ -222,31 +247,52 @@ static int cg_init(dgc_ipdg_s* s, IpdgLevel<T>& L, const T* b, double tol_rel2_t
       if (rc) return rc;
     }
   }
-  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kInit, L.scal, history, st));
-  // kInit sets beta = 0: the first iteration forms p = z + 0 p from the zeroed p
+  if (!fused) DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kInit, L.scal, history, st));
   return DGC_OK;
 }
 
-// One PCG iteration: pass 1 forms the search direction p' = z + beta p (into
+// One PCG iteration. Pass 1 forms the search direction p' = z + beta p (into
 // the other direction buffer) and applies the first operator pass to it;
-// pass 2 gives q = S p' and the partials of p'.q; alpha; the update
-// (x, r, z = P r) [+ the coarse correction]; beta and the stopping test.
+// pass 2 gives q = S p' and the partials of p'.q; the update forms x, r and
+// z = P r with the partials (r.z, r.r) [two-level: then the coarse
+// restriction, solve and prolongation, which rewrites the partials].
+// Unfused, the scalar kernel finishes alpha after pass 2 and beta, the history
+// and the stopping test after the update (5 kernels with block-Jacobi). Fused,
+// the update sums pass 2's partials itself for alpha, and the next
+// iteration's pass 1 sums the update's partials for beta and the stopping
+// test, reading record it % 2 and writing the other (3 kernels).
 // Iteration `it` of a solve reads direction buffer it % 2 and writes the other.
 template <typename T>
 static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int it) {
   hipStream_t st = s->disc->ctx->stream;
+  const bool fused = fused_iteration(s);
   T* pold = (it & 1) ? L.p2 : L.p;
   T* pnew = (it & 1) ? L.p : L.p2;
+  T* rin = L.rec + (it & 1) * dgc::kRecStride;
+  T* rout = L.rec + ((it + 1) & 1) * dgc::kRecStride;
+  T* scal = fused ? rout : L.scal;  // what the kernels after pass 1 read
   dgc::IpdgParams<T> pr = L.params(*s);
   pr.u = pold;
   pr.dirz = L.z;
   pr.dirp = pnew;
   pr.out = L.q;
   pr.dotv = pnew;
-  pr.partial = L.partial;
-  pr.scal = L.scal;
-  DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpd, st));
-  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterApply, L.scal, history, st));
+  if (fused) {
+    pr.scal = rin;
+    pr.rec_out = rout;
+    pr.upart = L.partial;
+    pr.nparts = s->grid;
+    pr.history = history;
+    DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kPhaseOne, st));
+    pr.scal = rout;
+    pr.partial = L.dpartial;
+    DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpdPhaseTwo, st));
+  } else {
+    pr.scal = L.scal;
+    pr.partial = L.partial;
+    DGC_LAUNCH(dgc::ipdg_apply<T>(s->disc->order, pr, dgc::kSpd, st));
+    DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterApply, L.scal, history, st));
+  }
   dgc::CgParams<T> cp{};
   cp.K = s->disc->K;
   cp.x = L.x;
@@ -255,16 +301,22 @@ static int cg_iteration(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int it)
   cp.q = L.q;
   cp.z = L.z;
   cp.P = L.use_P ? L.P : nullptr;
-  cp.scal = L.scal;
+  cp.scal = scal;
   cp.partial = L.partial;
+  if (fused) {
+    cp.dpart = L.dpartial;
+    cp.nparts = s->grid;
+    cp.rec_out = rout;
+  }
   DGC_LAUNCH(dgc::cg_update_launch<T>(s->disc->order, cp, st));
   if constexpr (std::is_same<T, double>::value) {
     if (L.use_coarse) {
-      int rc = coarse_correct(s, L.r, L.z, L.partial, 0, L.scal);
+      int rc = coarse_correct(s, L.r, L.z, L.partial, 0, scal);
       if (rc) return rc;
     }
   }
-  DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterUpdate, L.scal, history, st));
+  if (!fused)
+    DGC_LAUNCH(dgc::cg_scalars_launch<T>(L.partial, s->grid, dgc::kAfterUpdate, L.scal, history, st));
   return DGC_OK;
 }
 
@@ -296,9 +348,11 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
     L.graph_P = L.use_P;
     L.graph_coarse = L.use_coarse;
   }
+  // after whole chunks the current scalars are in record 0 (fused) or L.scal
+  const T* cur = fused_iteration(s) ? L.rec : L.scal;
   T h[dgc::kCgScalars];
   for (;;) {
-    DGC_HIP_TRY(hipMemcpyAsync(h, L.scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipMemcpyAsync(h, cur, sizeof(h), hipMemcpyDeviceToHost, st));
     DGC_HIP_TRY(hipStreamSynchronize(st));
     if (h[dgc::kDone] != T(0)) break;
     if (int(h[dgc::kIter]) >= maxit) break;
@@ -362,9 +416,12 @@ int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const d
   if (rc == DGC_OK) rc = s->f64.init(*s, drT, dsT, lT, mT, geo, J, tau);
   if (rc == DGC_OK) rc = s->f32.init(*s, drT, dsT, lT, mT, geo, J, tau);
   if (rc == DGC_OK) rc = alloc(&s->hist, 1);
+  if (rc == DGC_OK) rc = alloc(&s->dot, 257);
   if (rc != DGC_OK) {
     s->f64.release();
     s->f32.release();
+    (void)hipFree(s->hist);
+    (void)hipFree(s->dot);
     delete s;
     return rc;
   }
@@ -379,6 +436,7 @@ int dgc_ipdg_destroy(dgc_ipdg_t s) {
   s->f64.release();
   s->f32.release();
   (void)hipFree(s->hist);
+  (void)hipFree(s->dot);
   (void)hipFree(s->color);
   delete s;
   return DGC_OK;
@@ -621,6 +679,8 @@ int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell
     release_coarse(s);
     return rc;
   }
+  // the solve reads rc in 4-entry groups up to ld: the padding must be zero
+  DGC_HIP_TRY(hipMemset(cs.rc, 0, size_t(ld) * sizeof(double)));
   DGC_HIP_TRY(hipMemcpy(dptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(dagg, cell_agg.data(), cell_agg.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -761,11 +821,10 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
       break;
     }
     hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)L.q, L.r, n);
-    std::vector<double> hr(n);
-    DGC_HIP_TRY(hipMemcpyAsync(hr.data(), L.r, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    DGC_HIP_TRY(hipStreamSynchronize(st));
+    DGC_LAUNCH(dgc::dot_device(L.r, L.r, n, s->dot, s->dot + 256, st));
     double rr = 0.0;
-    for (double v : hr) rr += v * v;
+    DGC_HIP_TRY(hipMemcpyAsync(&rr, s->dot + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+    DGC_HIP_TRY(hipStreamSynchronize(st));
     rel = std::sqrt(rr / bb);
     if (!(rel == rel)) {
       rc = dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg: residual is not finite");

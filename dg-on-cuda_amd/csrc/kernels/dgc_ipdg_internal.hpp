@@ -23,8 +23,10 @@ struct IpdgLevel {
   T *qx = nullptr, *qy = nullptr, *du = nullptr;
   T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr;
   T* p2 = nullptr;  // second search-direction buffer (p alternates between p and p2)
-  T* partial = nullptr;
+  T* partial = nullptr;   // [2 * grid] partial sums
+  T* dpartial = nullptr;  // [grid] partials of p.q in the fused iteration
   T* scal = nullptr;
+  T* rec = nullptr;  // [2 * kRecStride] the fused iteration's alternating records
   T* P = nullptr;  // block-Jacobi inverses, symmetric, packed [Np(Np+1)/2][K]
   bool use_P = false;
   bool use_coarse = false;  // add the coarse-space correction (FP64 level only)
@@ -50,6 +52,7 @@ struct dgc_ipdg_s {
   int* color = nullptr;
   double* hist = nullptr;  // residual history (squared norms)
   int hist_len = 0;
+  double* dot = nullptr;   // [257] partials and result of a device dot product
   IpdgLevel<double> f64;
   IpdgLevel<float> f32;
   // coarse space of the two-level preconditioner (device arrays owned here)

@@ -215,6 +215,22 @@ def test_poisson_32k_n3_block_jacobi_pcg_to_1e10(ctx):
     assert err < 1e-4
 
 
+def test_pcg_above_the_fused_reduction_limit(ctx):
+    """Above 4096 partial sums (K > 262144 cells) the CG scalars are finished
+    by separate one-workgroup kernels instead of inside the iteration's
+    kernels: 363 x 363 squares (K = 263538), N = 1, block-Jacobi PCG to
+    1e-10 by the oracle's FP64 residual."""
+    mesh, disc, op, b, u_exact = _poisson_problem(ctx, 363, 1)
+    assert (disc.K + 63) // 64 > 4096
+    x = dgc_amd.DeviceArray(ctx, disc.size)
+    res = op.pcg(b, x, rtol=1e-10, maxit=40000, precond=1, history=True)
+    rr = _oracle_relres(mesh, disc, x.download(), b.download())
+    print(f"PCG(block-Jacobi) K={disc.K} N=1: {res['iters']} it, {res['ms']:.1f} ms, oracle relres {rr:.2e}")
+    assert res["relres"] <= 1e-10 and rr <= 2e-10
+    h = res["history"]
+    assert np.all(np.isfinite(h)) and h[-1] <= 1e-10 * np.linalg.norm(b.download())
+
+
 def test_poisson_mixed_precision_refinement(ctx):
     mesh, disc, op, b, u_exact = _poisson_problem(ctx, 64, 3)
     x = dgc_amd.DeviceArray(ctx, disc.size)
