@@ -345,6 +345,10 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
     out["speedup_learned_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["learned_fcg"]["median_ms"], 2)
     out["speedup_two_level_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 2)
     out["learned_vs_two_level_ms"] = round(out["learned_fcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 3)
+    out["iteration_bandwidth"] = {
+        name: cg_iteration_roofline(disc.K, N, out[name]["median_ms"], out[name]["median_iters"],
+                                    nc if name == "two_level_pcg" else 0)
+        for name in ("bj_pcg", "two_level_pcg")}
     if mixed:
         f, _ = dgc.poisson_rhs(xg, yg, seed=3)
         op.mass(dgc.DeviceArray.from_host(ctx, f), b)
@@ -356,6 +360,62 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
         print(f"#   {name}: median {out[name]['median_iters']} it, {out[name]['median_ms']:.2f} ms, "
               f"max relres {out[name]['max_relres']:.2e}", file=sys.stderr, flush=True)
     del net, op, disc
+    return out
+
+
+def ipdg_apply_bytes(K, N):
+    """Algorithmic bytes of one SPD apply (pass 1: read u, 13 geometry doubles
+    and 3 connectivity ints per cell, write q (2 Np) and the jumps (3 Nfp);
+    pass 2: read q, geometry, connectivity, penalties (3), jumps and J, write
+    out)."""
+    Np, Nfp = (N + 1) * (N + 2) // 2, N + 1
+    pass1 = 8 * Np + 13 * 8 + 12 + 16 * Np + 24 * Nfp
+    pass2 = 16 * Np + 13 * 8 + 12 + 24 + 24 * Nfp + 8 + 8 * Np
+    return K * (pass1 + pass2)
+
+
+def cg_iteration_roofline(K, N, ms, iters, nc=0):
+    """Algorithmic bytes per PCG iteration over its mean time: the apply with
+    the direction formed in pass 1 (reads z and the old direction, writes the
+    new one) and the dot operand in pass 2, the update (reads r, x, p, q and
+    the packed block inverses, writes x, r, z); two-level adds restriction
+    (r, 6 FP32 coefficients), the coarse solve (FP32 inverse, nc x nc) and the
+    prolongation (reads z, r, writes z)."""
+    Np = (N + 1) * (N + 2) // 2
+    b = ipdg_apply_bytes(K, N) + K * (8 * Np + 8 * Np + 8 * Np)  # z + new direction + dot operand
+    b += K * (32 * Np + 8 * Np * (Np + 1) // 2 + 24 * Np)
+    if nc:
+        b += K * (8 * Np + 24) + 4 * nc * nc + K * 24 * Np
+    us = ms * 1e3 / max(1, iters)
+    return dict(bytes_per_iteration=int(b), us_per_iteration=round(us, 2), GBs=round(b / us / 1e3, 1),
+                frac_of_8TBs=round(b / us / 1e3 / HBM_PEAK_GBS, 4))
+
+
+def ipdg_apply_bench(dgc, ctx, reps=30):
+    """The SPD IPDG apply (both passes) at N = 3 on 256 x 256 (K = 131072,
+    cache-resident) and 1024 x 1024 squares (K = 2,097,152: vectors of 168 MB,
+    HBM-resident), tau = (N+1)(N+2)/h, random input; wall time of `reps`
+    back-to-back applies on the solver's stream."""
+    out = []
+    for n_side in (256, 1024):
+        mesh = dgc.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=3)
+        disc = dgc.Discretization(ctx, mesh, 3)
+        op = dgc.IPDG(disc, tau_mode=1)
+        u = dgc.DeviceArray.from_host(ctx, np.random.default_rng(0).standard_normal(disc.size))
+        w = dgc.DeviceArray(ctx, disc.size)
+        for _ in range(3):
+            op.apply_spd(u, w)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.apply_spd(u, w)
+        ctx.sync()
+        us = (time.perf_counter() - t0) / reps * 1e6
+        b = ipdg_apply_bytes(disc.K, 3)
+        out.append(dict(K=disc.K, N=3, us_per_apply=round(us, 2), bytes_per_apply=int(b), GBs=round(b / us / 1e3, 1),
+                        frac_of_8TBs=round(b / us / 1e3 / HBM_PEAK_GBS, 4)))
+        print(f"# IPDG SPD apply K={disc.K} N=3: {us:.1f} us, {out[-1]['GBs']} GB/s", file=sys.stderr, flush=True)
+        del op, disc, u, w
     return out
 
 
@@ -528,6 +588,7 @@ def main():
 
     if not args.no_ipdg:
         full = args.ipdg_full
+        extra["ipdg_apply"] = ipdg_apply_bench(dgc, ctx)
         extra["ipdg"] = [
             ipdg_bench(dgc, ctx, 128, 3, 32, 4096 if full else 1024, 1, 256 if full else 24, mixed=True),
             ipdg_bench(dgc, ctx, 256, 2, 32, 4096 if full else 512, 1, 256 if full else 6),

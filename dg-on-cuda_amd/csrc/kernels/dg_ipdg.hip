@@ -35,6 +35,15 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
   return s;
 }
 
+// The operator tables are invariant while a kernel runs; read through the
+// constant address space their wave-uniform rows become scalar loads even
+// after the kernel's own global stores (which otherwise force vector loads,
+// holding every coefficient in vector registers).
+template <typename T>
+__device__ __forceinline__ const T __attribute__((address_space(4)))* table(const T* p) {
+  return reinterpret_cast<const T __attribute__((address_space(4)))*>(reinterpret_cast<uintptr_t>(p));
+}
+
 template <int N, typename T>
 __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   using Tr = Tri<N>;
@@ -167,8 +176,8 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
 #pragma unroll
   for (int k = 0; k < Np; ++k) {
     const T uk = us[k][lane];
-    const T* a = p.DrT + k * Npad + j0;
-    const T* b = p.DsT + k * Npad + j0;
+    const auto* a = table(p.DrT) + k * Npad + j0;
+    const auto* b = table(p.DsT) + k * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       dr[i] = fma(a[i], uk, dr[i]);
@@ -184,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
 #pragma unroll
   for (int m = 0; m < NF; ++m) {
     const T fx = lx[m][lane], fy = ly[m][lane];
-    const T* l = p.LT + m * Npad + j0;
+    const auto* l = table(p.LT) + m * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       qx[i] = fma(-l[i], fx, qx[i]);
@@ -323,8 +332,8 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
 #pragma unroll
   for (int k = 0; k < Np; ++k) {
     const T qk = qxs[k][lane], rk = qys[k][lane];
-    const T* a = p.DrT + k * Npad + j0;
-    const T* b = p.DsT + k * Npad + j0;
+    const auto* a = table(p.DrT) + k * Npad + j0;
+    const auto* b = table(p.DsT) + k * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       ax[i] = fma(a[i], qk, ax[i]);
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
 #pragma unroll
   for (int m = 0; m < NF; ++m) {
     const T f = lf[m][lane];
-    const T* l = p.LT + m * Npad + j0;
+    const auto* l = table(p.LT) + m * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) lap[i] = fma(-l[i], f, lap[i]);
   }
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
 #pragma unroll
   for (int k = 0; k < Np; ++k) {
     const T v = qxs[k][lane];
-    const T* mm = p.MT + k * Npad + j0;
+    const auto* mm = table(p.MT) + k * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) o[i] = fma(mm[i], v, o[i]);
   }
@@ -412,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_mass(const IpdgParams<T> p) {
 #pragma unroll
   for (int k = 0; k < Np; ++k) {
     const T v = fs[k][lane];
-    const T* mm = p.MT + k * Npad + j0;
+    const auto* mm = table(p.MT) + k * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) o[i] = fma(mm[i], v, o[i]);
   }

@@ -362,6 +362,29 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
   return DGC_OK;
 }
 
+// ||b - S x||^2 on the device (S x into q, b - S x into r); waits for the value.
+int true_residual2(dgc_ipdg_s* s, const double* b, const double* x, double* q, double* r, double* rr) {
+  hipStream_t st = s->disc->ctx->stream;
+  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = x;
+  pr.out = q;
+  DGC_LAUNCH(dgc::ipdg_apply<double>(s->disc->order, pr, dgc::kSpd, st));
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)q, r, n);
+  DGC_LAUNCH(hipGetLastError());
+  return dot2(s, r, r, rr);
+}
+
+// v.w on the device; waits for the value.
+int dot2(dgc_ipdg_s* s, const double* v, const double* w, double* out) {
+  hipStream_t st = s->disc->ctx->stream;
+  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  DGC_LAUNCH(dgc::dot_device(v, w, n, s->dot, s->dot + 256, st));
+  DGC_HIP_TRY(hipMemcpyAsync(out, s->dot + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
 // FP64 PCG restart from the current iterate (used by the flexible-CG fallback).
 int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, int maxit, double* history) {
   return cg_init<double>(s, s->f64, b, tol2_times_bb, maxit, false, history);
@@ -812,19 +835,8 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
     total += it;
     if (rc != DGC_OK) break;
     // true residual r = b - S x
-    dgc::IpdgParams<double> pr = L.params(*s);
-    pr.u = L.x;
-    pr.out = L.q;
-    hipError_t le = dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st);
-    if (le != hipSuccess) {
-      rc = dgc_set_error(DGC_ERR_LAUNCH, "true residual apply failed");
-      break;
-    }
-    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)L.q, L.r, n);
-    DGC_LAUNCH(dgc::dot_device(L.r, L.r, n, s->dot, s->dot + 256, st));
     double rr = 0.0;
-    DGC_HIP_TRY(hipMemcpyAsync(&rr, s->dot + 256, sizeof(double), hipMemcpyDeviceToHost, st));
-    DGC_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = true_residual2(s, b, L.x, L.q, L.r, &rr))) break;
     rel = std::sqrt(rr / bb);
     if (!(rel == rel)) {
       rc = dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg: residual is not finite");
@@ -865,17 +877,8 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
   IpdgLevel<double>& H = s->f64;
   IpdgLevel<float>& L = s->f32;
   L.use_P = true;
-  std::vector<double> hv(n);
-  auto norm2 = [&](const double* v, double& out) -> int {
-    DGC_HIP_TRY(hipMemcpyAsync(hv.data(), v, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    DGC_HIP_TRY(hipStreamSynchronize(st));
-    double a = 0.0;
-    for (double t : hv) a += t * t;
-    out = a;
-    return DGC_OK;
-  };
   double bb;
-  int rc = norm2(b, bb);
+  int rc = dot2(s, b, b, &bb);
   if (rc) return rc;
   DGC_REQUIRE(std::isfinite(bb), "dgc_ipdg_pcg_mixed: non-finite right-hand side");
   hipEvent_t e0, e1;
@@ -912,13 +915,7 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
     // x += ||r|| e;  r = b - S x (FP64)
     hipLaunchKernelGGL(add_scaled_float_kernel, dim3(grid_for(n)), dim3(256), 0, st, H.x, rn, (const float*)L.x, n);
     DGC_LAUNCH(hipGetLastError());
-    dgc::IpdgParams<double> pr = H.params(*s);
-    pr.u = H.x;
-    pr.out = H.q;
-    DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
-    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, 1.0, b, -1.0, (const double*)H.q, H.r, n);
-    DGC_LAUNCH(hipGetLastError());
-    if ((rc = norm2(H.r, rr))) return rc;
+    if ((rc = true_residual2(s, b, H.x, H.q, H.r, &rr))) return rc;
     rel = std::sqrt(rr / bb);
     ++outer;
     if (!(rel == rel)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg_mixed: residual is not finite");

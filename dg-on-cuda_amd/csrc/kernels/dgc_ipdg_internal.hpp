@@ -65,6 +65,11 @@ struct dgc_ipdg_s {
 // z += Phi A_c^{-1} Phi^T r with partials (r.z, r.r) or (z.z, r.r) into `partial`.
 int coarse_correct(dgc_ipdg_s* s, const double* r, double* z, double* partial, int zz, const double* scal);
 
+// ||b - S x||^2 (S x into q, b - S x into r) and v.w, on the device; both wait
+// for the value.
+int true_residual2(dgc_ipdg_s* s, const double* b, const double* x, double* q, double* r, double* rr);
+int dot2(dgc_ipdg_s* s, const double* v, const double* w, double* out);
+
 // FP64 block-Jacobi PCG continuing from the current iterate f64.x.
 int cg_init_fp64_restart(dgc_ipdg_s* s, const double* b, double tol2_times_bb, int maxit, double* history);
 int cg_run_fp64(dgc_ipdg_s* s, double* history, int maxit, int* iters);

@@ -694,17 +694,10 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   // after a stall, or restart it while recurrence drift leaves the true
   // residual above rtol
   auto true_rel = [&](double& rel) -> int {
-    dgc::IpdgParams<double> pr = Lv.params(*s);
-    pr.u = Lv.x;
-    pr.out = Lv.q;
-    DGC_LAUNCH(dgc::ipdg_apply<double>(d->order, pr, dgc::kSpd, st));
-    std::vector<double> hq(n);
-    DGC_HIP_TRY(hipMemcpyAsync(hq.data(), Lv.q, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    DGC_HIP_TRY(hipStreamSynchronize(st));
     double rr = 0.0;
-    for (size_t i = 0; i < n; ++i) rr += (hb[i] - hq[i]) * (hb[i] - hq[i]);
+    const int r3 = true_residual2(s, b, Lv.x, Lv.q, Lv.r, &rr);
     rel = std::sqrt(rr / bb);
-    return DGC_OK;
+    return r3;
   };
   double rel = 1.0;
   if ((rc = true_rel(rel))) return rc;
