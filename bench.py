@@ -364,13 +364,13 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
 
 
 def ipdg_apply_bytes(K, N):
-    """Algorithmic bytes of one SPD apply (pass 1: read u, 13 geometry doubles
-    and 3 connectivity ints per cell, write q (2 Np) and the jumps (3 Nfp);
-    pass 2: read q, geometry, connectivity, penalties (3), jumps and J, write
-    out)."""
+    """Algorithmic bytes of one SPD apply (pass 1: read u, the 4 metric terms
+    and 3 connectivity ints per cell (normals and face_J / J follow from the
+    metric terms), write q (2 Np) and the jumps (3 Nfp); pass 2: read q, the
+    metric terms, connectivity, penalties (3), jumps and J, write out)."""
     Np, Nfp = (N + 1) * (N + 2) // 2, N + 1
-    pass1 = 8 * Np + 13 * 8 + 12 + 16 * Np + 24 * Nfp
-    pass2 = 16 * Np + 13 * 8 + 12 + 24 + 24 * Nfp + 8 + 8 * Np
+    pass1 = 8 * Np + 4 * 8 + 12 + 16 * Np + 24 * Nfp
+    pass2 = 16 * Np + 4 * 8 + 12 + 24 + 24 * Nfp + 8 + 8 * Np
     return K * (pass1 + pass2)
 
 

@@ -44,6 +44,17 @@ __device__ __forceinline__ const T __attribute__((address_space(4)))* table(cons
   return reinterpret_cast<const T __attribute__((address_space(4)))*>(reinterpret_cast<uintptr_t>(p));
 }
 
+// Outward normal of face e scaled by face_J / J, from the metric terms of a
+// straight-sided cell: -(sx, sy) on face 0, (rx + sx, ry + sy) on face 1,
+// -(rx, ry) on face 2. Replaces reading nx, ny and face_J / J per face.
+template <typename T>
+__device__ __forceinline__ void face_vector(int e, T rx, T ry, T sx, T sy, T& vx, T& vy) {
+  const T ax = e == 0 ? sx : (e == 1 ? rx + sx : rx);
+  const T ay = e == 0 ? sy : (e == 1 ? ry + sy : ry);
+  vx = e == 1 ? ax : -ax;
+  vy = e == 1 ? ay : -ay;
+}
+
 template <int N, typename T>
 __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   using Tr = Tri<N>;
@@ -83,19 +94,10 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
     ownz[i] = ok ? raw_z((size_t)j * K + c) : T(0);
   }
   int code[FPW];
-  T fnx[FPW], fny[FPW], ffs[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
     const int m = wave + kWaves * i;
-    const int e = m / Nfp;
-    code[i] = kBoundaryCode;
-    fnx[i] = fny[i] = ffs[i] = 0;
-    if (valid && m < NF) {
-      code[i] = p.nbr[(size_t)e * K + c];
-      fnx[i] = p.geo[(size_t)(kNx0 + e) * K + c];
-      fny[i] = p.geo[(size_t)(kNy0 + e) * K + c];
-      ffs[i] = p.geo[(size_t)(kFs0 + e) * K + c];
-    }
+    code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
   }
   T rx = 0, ry = 0, sx = 0, sy = 0;
   if (valid) {
@@ -164,8 +166,10 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
       }
       p.du[(size_t)m * K + c] = du;
     }
-    lx[m][lane] = fnx[i] * du * ffs[i] / T(2);
-    ly[m][lane] = fny[i] * du * ffs[i] / T(2);
+    T vx, vy;  // n face_J / J
+    face_vector(e, rx, ry, sx, sy, vx, vy);
+    lx[m][lane] = vx * du / T(2);
+    ly[m][lane] = vy * du / T(2);
   }
   __syncthreads();
 
@@ -242,18 +246,15 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
     owny[i] = ok ? p.qy[(size_t)j * K + c] : T(0);
   }
   int code[FPW];
-  T fnx[FPW], fny[FPW], ffs[FPW], ftau[FPW], fdu[FPW];
+  T ftau[FPW], fdu[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
     const int m = wave + kWaves * i;
     const int e = m / Nfp;
     code[i] = kBoundaryCode;
-    fnx[i] = fny[i] = ffs[i] = ftau[i] = fdu[i] = 0;
+    ftau[i] = fdu[i] = 0;
     if (valid && m < NF) {
       code[i] = p.nbr[(size_t)e * K + c];
-      fnx[i] = p.geo[(size_t)(kNx0 + e) * K + c];
-      fny[i] = p.geo[(size_t)(kNy0 + e) * K + c];
-      ffs[i] = p.geo[(size_t)(kFs0 + e) * K + c];
       ftau[i] = p.tau[(size_t)e * K + c];
       fdu[i] = p.du[(size_t)m * K + c];
     }
@@ -304,6 +305,9 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
     const int own = Tr::fnode(e, d);
     T v = 0;
     if (valid) {
+      T vx, vy;  // n face_J / J
+      face_vector(e, rx, ry, sx, sy, vx, vy);
+      const T fs = sqrt(vx * vx + vy * vy);
       const int nf = code[i] & 3, idx = code[i] >> 2;
       T dqn = 0;
       if (nf != kBoundaryCode) {
@@ -317,9 +321,9 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
           bx = farx[i];
           by = fary[i];
         }
-        dqn = (qxs[own][lane] - bx) * fnx[i] + (qys[own][lane] - by) * fny[i];
+        dqn = (qxs[own][lane] - bx) * vx + (qys[own][lane] - by) * vy;  // n.[q] face_J / J
       }
-      v = (dqn + T(2) * (ftau[i] * fdu[i])) * ffs[i] / T(2);
+      v = (dqn + T(2) * (ftau[i] * fdu[i]) * fs) / T(2);
     }
     lf[m][lane] = v;
   }
