@@ -104,8 +104,9 @@ def mfma_stats(r, peak=MFMA_PEAK_F32_TFLOPS):
     N, Np, K = r["N"], r["Np"], r["K"]
     NF = 3 * (N + 1)
     MB, KS, KL = (Np + 15) // 16, (Np + 3) // 4, (NF + 3) // 4
-    issued = 2.0 * 1024 * MB * (2 * KS + KL) * ((K + 15) // 16)
-    useful = 2.0 * (2 * Np * Np + Np * NF) * K
+    F = r.get("F", 1)  # fields: each gets the Dr, Ds and LIFT products
+    issued = 2.0 * 1024 * MB * (2 * KS + KL) * ((K + 15) // 16) * F
+    useful = 2.0 * (2 * Np * Np + Np * NF) * K * F
     dur = r["tune"]["ms"][r["tune"]["choice"]] * 1e-3
     return dict(issued_tflops=round(issued / dur / 1e12, 3), useful_tflops=round(useful / dur / 1e12, 3),
                 peak_tflops=peak, frac_issued=round(issued / dur / 1e12 / peak, 4))
@@ -481,6 +482,26 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     return out
 
 
+def maxwell_fp32_form(dgc, ctx, steps=50, warmup=5):
+    """Maxwell TM on the sweep's 256 x 256 periodic lattice (K = 131072),
+    N = 4..7, Ez = sin(pi x) sin(pi y): the FP64 path (autotuned tiling)
+    against the FP32-contraction form, rate and relative L2 difference of Ez
+    after the same steps."""
+    mesh = dgc.Mesh.structured(256, 256, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    out = []
+    for N in (4, 5, 6, 7):
+        r = run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=2, physics="maxwell")
+        q = run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=2, physics="maxwell", precision="fp32-mfma")
+        err = float(np.linalg.norm(q["u"] - r["u"]) / np.linalg.norm(r["u"]))
+        out.append(dict(N=N, fp64=r["value"], fp64_tiling=r["tune"]["choice"], fp32_mfma=q["value"],
+                        speedup=round(q["value"] / r["value"], 3), rel_l2_vs_fp64=err,
+                        fp32_mfma_matrix=mfma_stats(q)))
+        print(f"# Maxwell N={N}: fp64 {r['value'] / 1e9:.1f} G, fp32-mfma {q['value'] / 1e9:.1f} G "
+              f"({out[-1]['speedup']}x), rel L2 {err:.1e}", file=sys.stderr, flush=True)
+        del r, q
+    return out
+
+
 def hbm_resident(dgc, ctx, steps, warmup):
     """Low-order FP64 RK4 with every vector in HBM: 2048 x 2048 periodic
     squares (K = 8,388,608, jitter 0.05h, seed 1), advection (a = (1, 0.5),
@@ -585,6 +606,8 @@ def main():
     extra = {}
     if not args.no_hbm:
         extra["hbm_resident"] = hbm_resident(dgc, ctx, max(10, args.steps // 20), 2)
+    if not args.no_fp32:
+        extra["maxwell_fp32_mfma"] = maxwell_fp32_form(dgc, ctx)
 
     if not args.no_ipdg:
         full = args.ipdg_full

@@ -41,7 +41,7 @@ int default_waves_per_tile(int order, int physics) {
 }
 
 hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s) {
-  if (p.opF != nullptr) return physics == kAdvection ? launch_stage_mfma(order, physics, stage, p, s) : hipErrorInvalidValue;
+  if (p.opF != nullptr) return launch_stage_mfma(order, physics, stage, p, s);
   if (wt == kMfmaTiling) {
     if (p.opD == nullptr) return hipErrorInvalidValue;
     return launch_stage_mfma(order, physics, stage, p, s);

@@ -1,6 +1,6 @@
 // Launcher of the matrix-core RK4 stage (kernels and tables in
-// dg_stage_mfma.hpp): the FP32-contraction form of the advection operator and
-// the FP64 matrix-core variant of both operators. The nontemporal-stream
+// dg_stage_mfma.hpp): the FP32-contraction form and the FP64 matrix-core
+// variant, both operators. The nontemporal-stream
 // instantiations live in dg_rk_mfma_nt.hip.
 #include "dg_stage_mfma.hpp"
 

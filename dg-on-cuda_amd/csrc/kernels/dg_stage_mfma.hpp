@@ -2,7 +2,7 @@
 // the advection (1 field) and Maxwell TM (3 fields) operators.
 //
 // Compute type CT = float: the FP32-contraction form (v_mfma_f32_16x16x4f32,
-// operands rounded to FP32). CT = double: a variant of the plain FP64 path
+// operands rounded to FP32), both operators. CT = double: a variant of the plain FP64 path
 // (v_mfma_f64_16x16x4f64) that the autotuner can pick. The state, the stage
 // accumulation, the metric terms, the numerical fluxes and the stage update
 // are FP64 in both.
@@ -326,11 +326,13 @@ StageFn pick_order(int order, int stage) {
   }
 }
 
-// Kernel of the FP32-contraction form (f32, advection only) or of the FP64
-// matrix-core variant (both operators); nullptr if not built.
+// Kernel of the FP32-contraction form (f32) or of the FP64 matrix-core
+// variant, for either operator; nullptr if not built.
 template <bool NT>
 StageFn pick(int order, int physics, bool f32, int stage) {
-  if (f32) return physics == kAdvection ? pick_order<float, kAdvection, NT>(order, stage) : nullptr;
+  if (f32)
+    return physics == kMaxwellTM ? pick_order<float, kMaxwellTM, NT>(order, stage)
+                                 : pick_order<float, kAdvection, NT>(order, stage);
   return physics == kMaxwellTM ? pick_order<double, kMaxwellTM, NT>(order, stage)
                                : pick_order<double, kAdvection, NT>(order, stage);
 }

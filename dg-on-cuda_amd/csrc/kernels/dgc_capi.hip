@@ -627,8 +627,6 @@ int dgc_op_waves_per_tile(dgc_op_t op) { return op ? op->wt : 0; }
 int dgc_op_set_precision(dgc_op_t op, int precision) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_precision: null operator");
   DGC_REQUIRE(precision == 0 || precision == 1, "dgc_op_set_precision: precision must be 0 (FP64) or 1 (FP32 MFMA)");
-  DGC_REQUIRE(precision == 0 || op->physics == dgc::kAdvection,
-              "dgc_op_set_precision: the FP32 matrix-core form is built for the advection operator");
   op->precision = precision;
   if (op->graph) {
     (void)hipGraphExecDestroy(op->graph);

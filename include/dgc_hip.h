@@ -131,7 +131,7 @@ int dgc_op_waves_per_tile(dgc_op_t op);
  * 0: FP64 throughout (default). 1: Dr u, Ds u and LIFT flux on the matrix
  * cores in FP32 (16x16x4 f32 MFMA, operands rounded to FP32), with the state,
  * stage accumulation, metric terms, fluxes and the stage update in FP64.
- * Form 1 is built for the advection operator. */
+ * Both operators (Maxwell TM and advection); an ensemble takes FP64 operators. */
 int dgc_op_set_precision(dgc_op_t op, int precision);
 int dgc_op_precision(dgc_op_t op);
 /* Cache policy of the RK stage kernels (no reference counterpart). The RK4

@@ -1,6 +1,7 @@
-"""FP32 matrix-core form of the advection stage (Dr u, Ds u and LIFT flux in
-FP32 on 16x16x4 MFMA; state, metric terms, fluxes and the RK4 update in
-FP64) against the oracle (the reference's FP64 CPU path).
+"""FP32 matrix-core form of the RK stage (Dr u, Ds u and LIFT flux in FP32 on
+16x16x4 MFMA; state, metric terms, fluxes and the RK4 update in FP64), for
+the advection and Maxwell TM operators, against the oracle (the reference's
+FP64 CPU path).
 
 Stated tolerance of the FP32-contraction form: relative L2 <= 1e-5 against
 the FP64 oracle for one RHS evaluation and for 20 RK4 steps, every order
@@ -92,9 +93,49 @@ def test_inflow_boundary_and_tail_tile(ctx, golden):
         assert rel(n2c(dout[0].download(), K, Np), want) < TOL_FP32
 
 
-def test_maxwell_rejects_the_fp32_form(ctx):
-    mesh = dgc_amd.Mesh.structured(4, 4, -1, 1, -1, 1)
-    op = dgc_amd.MaxwellTM(dgc_amd.Discretization(ctx, mesh, 2))
-    with pytest.raises(dgc_amd.DGCError):
-        op.precision = "fp32-mfma"
-    assert op.precision == "fp64"
+@pytest.mark.parametrize("N", range(1, 8))
+def test_maxwell_rhs_and_rk4_vs_oracle(ctx, N):
+    """Maxwell TM (three fields, PEC walls, upwind flux) in the FP32 form: one
+    RHS and 10 RK4 steps against the oracle, same stated tolerance."""
+    mesh = dgc_amd.Mesh.structured(10, 10, -1, 1, -1, 1, jitter=0.1, seed=6)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.MaxwellTM(disc)
+    op.precision = "fp32-mfma"
+    assert op.precision == "fp32-mfma"
+    K, Np = disc.K, disc.Np
+    u0 = np.random.default_rng(N).standard_normal(3 * K * Np)
+    P = Problem.from_mesh(mesh, N)
+    up = lambda u: [dgc_amd.DeviceArray.from_host(ctx, c2n(f, K, Np)) for f in np.split(u, 3)]
+    down = lambda a: np.concatenate([n2c(x.download(), K, Np) for x in a])
+    out = [dgc_amd.DeviceArray(ctx, K * Np) for _ in range(3)]
+    op.rhs(up(u0), 0.0, out)
+    err_rhs = rel(down(out), P.maxwell_rhs(u0, 0.0))
+    dt = 0.5 * mesh.min_edge / (N + 1) ** 2
+    state = up(u0)
+    op.rk4_steps(state, 0.0, dt, 10, op.new_work())
+    want, _ = P.rk4(0, u0, 0.0, dt, nsteps=10)
+    err = rel(down(state), want)
+    print(f"Maxwell N={N}: fp32-mfma rhs rel L2 {err_rhs:.2e}, 10 RK4 steps {err:.2e}")
+    assert 1e-12 < err_rhs < TOL_FP32
+    assert err < TOL_FP32
+
+
+def test_maxwell_cavity_fp32_form(ctx, golden):
+    """The Maxwell cavity (K = 512, N = 4, to t = 1) in the FP32 form against
+    the reference's golden final state."""
+    fix = golden("maxwell_c1.npz")
+    N = int(fix["order"])
+    mesh = dgc_amd.Mesh.from_arrays(fix["vx"], fix["vy"], fix["etov"])
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.MaxwellTM(disc)
+    op.precision = "fp32-mfma"
+    K, Np = disc.K, disc.Np
+    n = K * Np
+    u0 = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    state = [dgc_amd.DeviceArray.from_host(ctx, c2n(f, K, Np)) for f in np.split(u0, 3)]
+    t, steps = op.rk4_run(state, 0.0, 1.0, float(fix["dt"]), op.new_work())
+    assert steps == int(fix["steps"])
+    got = np.concatenate([n2c(a.download(), K, Np) for a in state])
+    err = rel(got, fix["final"])
+    print(f"cavity fp32-mfma: {steps} steps, rel L2 vs reference {err:.2e}")
+    assert err < TOL_FP32
