@@ -21,16 +21,17 @@
 
 namespace dgc {
 
-template <typename T>
+// Sum over the W waves of a workgroup: wave reduction, then the wave totals
+// in wave order; valid in thread 0.
+template <int W = kWaves, typename T>
 __device__ __forceinline__ T block_sum(T v, T* red) {
-  // wave reduction then across the 4 waves, fixed order
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) red[wave] = v;
   __syncthreads();
   T s = 0;
   if (threadIdx.x == 0) {
-    for (int w = 0; w < kWaves; ++w) s += red[w];
+    for (int w = 0; w < W; ++w) s += red[w];
   }
   return s;
 }
@@ -55,13 +56,15 @@ __device__ __forceinline__ void face_vector(int e, T rx, T ry, T sx, T sy, T& vx
   vy = e == 1 ? ay : -ay;
 }
 
-template <int N, typename T>
-__global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
+// W waves serve a 64-cell tile (lane = cell): 4 (256 threads) or 8 (512
+// threads, for small meshes: twice the waves in flight, half the rows each).
+template <int N, typename T, int W>
+__global__ __launch_bounds__(64 * W) void ipdg_pass1(const IpdgParams<T> p) {
   using Tr = Tri<N>;
   constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp;
-  constexpr int RPW = (Np + kWaves - 1) / kWaves;
-  constexpr int Npad = RPW * kWaves;
-  constexpr int FPW = (NF + kWaves - 1) / kWaves;  // face slots per wave (m = wave + 4 i)
+  constexpr int RPW = (Np + W - 1) / W;
+  constexpr int Npad = ipdg_rows(Np);
+  constexpr int FPW = (NF + W - 1) / W;  // face slots per wave (m = wave + W i)
   __shared__ T us[Np][kTile];
   __shared__ T lx[NF][kTile];
   __shared__ T ly[NF][kTile];
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   T own[RPW], ownz[RPW];
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wave + kWaves * i;
+    const int j = wave + W * i;
     const bool ok = valid && j < Np;
     own[i] = ok ? p.u[(size_t)j * K + c] : T(0);
     ownz[i] = ok ? raw_z((size_t)j * K + c) : T(0);
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   int code[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
   }
   T rx = 0, ry = 0, sx = 0, sy = 0;
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   T far[FPW], farz[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     const int d = m % Nfp;
     const int nf = code[i] & 3, idx = code[i] >> 2;
     far[i] = farz[i] = 0;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
   }
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wave + kWaves * i;
+    const int j = wave + W * i;
     if (j < Np) {
       us[j][lane] = own[i];
       if (p.dirz && valid) p.dirp[(size_t)j * K + c] = own[i];
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
 
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const T a = us[Tr::fnode(e, d)][lane];
@@ -216,16 +219,16 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass1(const IpdgParams<T> p) {
 
 // mode kLaplace: out = Lap u; mode kSpd: out = -J M Lap u (the SPD form),
 // optionally with partial[block] = sum over the tile of dotv . out.
-template <int N, typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
+template <int N, typename T, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
   using Tr = Tri<N>;
   constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp;
-  constexpr int RPW = (Np + kWaves - 1) / kWaves;
-  constexpr int Npad = RPW * kWaves;
+  constexpr int RPW = (Np + W - 1) / W;
+  constexpr int Npad = ipdg_rows(Np);
   __shared__ T qxs[Np][kTile];
   __shared__ T qys[Np][kTile];
   __shared__ T lf[NF][kTile];
-  __shared__ T red[kWaves];
+  __shared__ T red[W];
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
   const int wave = uniform(threadIdx.x >> 6);
@@ -233,14 +236,14 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
   const int c = c0 + lane;
   const bool valid = c < K;
   if (p.scal && p.scal[kDone] != T(0)) return;
-  constexpr int FPW = (NF + kWaves - 1) / kWaves;  // face slots per wave (m = wave + 4 i)
+  constexpr int FPW = (NF + W - 1) / W;  // face slots per wave (m = wave + W i)
 
   // ---- every independent read first: q rows, connectivity, face terms,
   // penalties, jumps and metric terms; then the neighbours' q outside the tile
   T ownx[RPW], owny[RPW];
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wave + kWaves * i;
+    const int j = wave + W * i;
     const bool ok = valid && j < Np;
     ownx[i] = ok ? p.qx[(size_t)j * K + c] : T(0);
     owny[i] = ok ? p.qy[(size_t)j * K + c] : T(0);
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
   T ftau[FPW], fdu[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     const int e = m / Nfp;
     code[i] = kBoundaryCode;
     ftau[i] = fdu[i] = 0;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
   T farx[FPW], fary[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     const int d = m % Nfp;
     const int nf = code[i] & 3, idx = code[i] >> 2;
     farx[i] = fary[i] = 0;
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
   }
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wave + kWaves * i;
+    const int j = wave + W * i;
     if (j < Np) {
       qxs[j][lane] = ownx[i];
       qys[j][lane] = owny[i];
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
 
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wave + kWaves * i;
+    const int m = wave + W * i;
     if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const int own = Tr::fnode(e, d);
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_pass2(const IpdgParams<T> p) {
     }
   }
   if (p.partial) {
-    const T s = block_sum(dot, red);
+    const T s = block_sum<W>(dot, red);
     if (threadIdx.x == 0) p.partial[blockIdx.x] = s;
   }
 }
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void ipdg_mass(const IpdgParams<T> p) {
   using Tr = Tri<N>;
   constexpr int Np = Tr::Np;
   constexpr int RPW = (Np + kWaves - 1) / kWaves;
-  constexpr int Npad = RPW * kWaves;
+  constexpr int Npad = ipdg_rows(Np);
   __shared__ T fs[Np][kTile];
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
@@ -557,15 +560,23 @@ __global__ __launch_bounds__(kBlock) void cg_scalars(const T* __restrict__ parti
 // ---------------------------------------------------------------------------
 namespace {
 
+template <typename T, int N, int W>
+void launch_passes(const IpdgParams<T>& p, int mode, int grid, hipStream_t s) {
+  if (mode != kPhaseTwo && mode != kSpdPhaseTwo)
+    hipLaunchKernelGGL((ipdg_pass1<N, T, W>), dim3(grid), dim3(64 * W), 0, s, p);
+  if (mode == kLaplace || mode == kPhaseTwo)
+    hipLaunchKernelGGL((ipdg_pass2<N, T, kLaplace, W>), dim3(grid), dim3(64 * W), 0, s, p);
+  else if (mode == kSpd || mode == kSpdPhaseTwo)
+    hipLaunchKernelGGL((ipdg_pass2<N, T, kSpd, W>), dim3(grid), dim3(64 * W), 0, s, p);
+}
+
 template <typename T, int N>
 hipError_t launch_apply_n(const IpdgParams<T>& p, int mode, hipStream_t s) {
   const int grid = (p.K + kTile - 1) / kTile;
-  if (mode != kPhaseTwo && mode != kSpdPhaseTwo)
-    hipLaunchKernelGGL((ipdg_pass1<N, T>), dim3(grid), dim3(kBlock), 0, s, p);
-  if (mode == kLaplace || mode == kPhaseTwo)
-    hipLaunchKernelGGL((ipdg_pass2<N, T, kLaplace>), dim3(grid), dim3(kBlock), 0, s, p);
-  else if (mode == kSpd || mode == kSpdPhaseTwo)
-    hipLaunchKernelGGL((ipdg_pass2<N, T, kSpd>), dim3(grid), dim3(kBlock), 0, s, p);
+  if (grid <= kIpdgWideTiles)
+    launch_passes<T, N, 8>(p, mode, grid, s);
+  else
+    launch_passes<T, N, 4>(p, mode, grid, s);
   return hipGetLastError();
 }
 

@@ -23,6 +23,14 @@ constexpr int kRecStride = 16;
 // Largest partial-sum count for which the fused iteration's kernels finish
 // the reductions themselves (every workgroup sums all partials, from cache).
 constexpr int kFusedMaxParts = 4096;
+// Row stride of the IPDG operator tables (transposed, zero-padded): a multiple
+// of 8, so that 4 or 8 waves per tile can each take a contiguous row block.
+__host__ __device__ constexpr int ipdg_rows(int Np) { return 8 * ((Np + 7) / 8); }
+// Meshes of at most this many 64-cell tiles run the operator passes with 8
+// waves per tile (more waves in flight), larger ones with 4. Measured with
+// block-Jacobi PCG: 8 waves 2.4 % faster at 512 tiles (K = 32768, N = 3),
+// 4 waves 8-9 % faster at 2048 tiles (K = 131072, N = 2 and 3).
+constexpr int kIpdgWideTiles = 1024;
 enum CgStep { kInit = 0, kAfterApply = 1, kAfterUpdate = 2 };
 
 template <typename T>
@@ -79,24 +87,26 @@ struct CgParams {
 };
 
 // Fixed-order sum of W interleaved series partial[W i + w], i < nparts, in
-// double, over a 256-thread workgroup; every thread receives the sums. Each
-// thread adds a strided subset in index order, waves combine by butterfly
-// shuffles (identical in every lane: IEEE addition commutes), wave totals
-// are added in wave order. Every workgroup, and the one-workgroup scalar
-// kernel, gets bitwise the same sums.
+// double, by the first 256 threads of a workgroup of 256 or more; every
+// thread receives the sums. Each of those threads adds a strided subset in
+// index order, waves combine by butterfly shuffles (identical in every lane:
+// IEEE addition commutes), the four wave totals are added in wave order.
+// Every workgroup of any kernel, and the one-workgroup scalar kernel, gets
+// bitwise the same sums.
 template <int W, typename T>
 __device__ __forceinline__ void sum_partials(const T* partial, int nparts, double (&out)[W]) {
   __shared__ double wsum[W][kWaves];
   double a[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) a[w] = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kBlock)
+  if (threadIdx.x < kBlock)
+    for (int i = threadIdx.x; i < nparts; i += kBlock)
 #pragma unroll
-    for (int w = 0; w < W; ++w) a[w] += double(partial[W * i + w]);
+      for (int w = 0; w < W; ++w) a[w] += double(partial[W * i + w]);
 #pragma unroll
   for (int w = 0; w < W; ++w)
     for (int off = 32; off > 0; off >>= 1) a[w] += __shfl_xor(a[w], off, 64);
-  if ((threadIdx.x & 63) == 0)
+  if ((threadIdx.x & 63) == 0 && threadIdx.x < kBlock)
 #pragma unroll
     for (int w = 0; w < W; ++w) wsum[w][threadIdx.x >> 6] = a[w];
   __syncthreads();

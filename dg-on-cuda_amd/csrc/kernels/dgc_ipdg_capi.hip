@@ -51,7 +51,7 @@ int upload(T** p, const std::vector<T>& h) {
 
 // Transposed, row-padded operator layout read by the tile kernels.
 std::vector<double> transpose_pad(const double* A, int rows, int cols) {
-  const int Npad = dgc::padded_rows(rows);
+  const int Npad = dgc::ipdg_rows(rows);
   std::vector<double> t(size_t(cols) * Npad, 0.0);
   for (int j = 0; j < rows; ++j)
     for (int k = 0; k < cols; ++k) t[size_t(k) * Npad + j] = A[size_t(j) * cols + k];
