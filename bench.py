@@ -539,6 +539,8 @@ def main():
     ap.add_argument("--hbm-only", action="store_true",
                     help="run only the HBM-resident N = 3 case (for counter collection)")
     ap.add_argument("--hbm-physics", default="advection", choices=("advection", "maxwell"))
+    ap.add_argument("--ipdg-apply-only", action="store_true",
+                    help="run only the IPDG SPD apply measurement (for kernel traces and counters)")
     ap.add_argument("--calib", action="store_true",
                     help="also run dgc_axpy over 2^27 doubles: a kernel of known traffic "
                          "(reads 2 GiB, writes 1 GiB) to scale HBM byte counters")
@@ -567,6 +569,9 @@ def main():
         ctx.sync()
         print("# calibration: 3 x dgc_axpy over 2^27 doubles done", file=sys.stderr, flush=True)
         del x, y, o
+    if args.ipdg_apply_only:
+        print(json.dumps(dict(workload="ipdg spd apply N=3", ipdg_apply=ipdg_apply_bench(dgc, ctx))))
+        return
     if args.hbm_only:
         hm = dgc.Mesh.structured(2048, 2048, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
         r = run_order(dgc, ctx, hm, 3, args.steps, args.warmup, physics=args.hbm_physics)

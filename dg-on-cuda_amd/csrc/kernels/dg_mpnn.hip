@@ -31,6 +31,25 @@ __device__ __forceinline__ void load_bf16x4(const uint16_t* p, float out[4]) {
   for (int v = 0; v < 4; ++v) out[v] = bf2f(uint16_t(u >> (16 * v)));
 }
 
+// Parameter tables are invariant while a kernel runs and read at
+// wave-uniform addresses: through the constant address space they become
+// scalar loads instead of per-lane vector loads held in vector registers.
+// bf16 weights are read in pairs (entries 2i, 2i+1; the tables start at even
+// offsets of the 256-byte-aligned parameter buffer).
+using ConstU32 = const uint32_t __attribute__((address_space(4)));
+using ConstF32 = const float __attribute__((address_space(4)));
+__device__ __forceinline__ ConstU32* weight_pairs(const uint16_t* w) {
+  return reinterpret_cast<ConstU32*>(reinterpret_cast<uintptr_t>(w));
+}
+__device__ __forceinline__ ConstF32* weight_f32(const float* w) {
+  return reinterpret_cast<ConstF32*>(reinterpret_cast<uintptr_t>(w));
+}
+// Entry i of a bf16 table read as pairs.
+__device__ __forceinline__ float weight_at(ConstU32* w, int i) {
+  const uint32_t u = w[i >> 1];
+  return __uint_as_float((i & 1) ? (u & 0xffff0000u) : (u << 16));
+}
+
 template <typename T>
 __device__ __forceinline__ T block_sum256(T v, T* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -116,7 +135,7 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, 
     acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 0) * 64 + lane]), acc0);
     acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 1) * 64 + lane]), acc1);
   }
-  const float* bias = net.wf + net.L.ob[layer];
+  ConstF32* bias = weight_f32(net.wf + net.L.ob[layer]);
   const int j = lane & 31;
   const float b0 = bias[j], b1 = bias[32 + j];
 #pragma unroll
@@ -145,42 +164,19 @@ __global__ __launch_bounds__(256) void out_kernel(MpnnNet net, MpnnOutArgs a, co
     if (valid) load_bf16x4(a.h3 + (size_t)c * kHid + 4 * q4, t);
     for (int v = 0; v < 4; ++v) h[4 * q4 + v] = t[v];
   }
-  const uint16_t* Wo = net.wbf + net.L.oWo;
-  const float* bo = net.wf + net.L.obo;
+  ConstU32* Wo = weight_pairs(net.wbf + net.L.oWo);
+  ConstF32* bo = weight_f32(net.wf + net.L.obo);
   float y[Np];
 #pragma unroll
   for (int j = 0; j < Np; ++j) {
     float acc = bo[j];
 #pragma unroll
-    for (int o = 0; o < kHid; ++o) acc = fmaf(bf2f(Wo[j * kHid + o]), h[o], acc);
+    for (int o = 0; o < kHid; ++o) acc = fmaf(weight_at(Wo, j * kHid + o), h[o], acc);
     y[j] = acc;
   }
   if (MODE == kOutY) {
     if (valid)
       for (int j = 0; j < Np; ++j) a.y[(size_t)c * Np + j] = y[j];
-    return;
-  }
-  if (MODE == kOutPrecond) {
-    const double scale = a.dscal[kSz] * a.dscal[kCt];
-    double rz = 0.0, zq = 0.0;
-    if (valid) {
-#pragma unroll
-      for (int j = 0; j < Np; ++j) {
-        const size_t idx = (size_t)j * K + c;
-        const double z = a.zb[idx] + scale * double(y[j]);
-        a.z[idx] = z;
-        if (a.r) rz += a.r[idx] * z;
-        if (a.q) zq += z * a.q[idx];
-      }
-    }
-    if (a.partial) {
-      const double s1 = block_sum256(rz, red);
-      const double s2 = block_sum256(zq, red);
-      if (threadIdx.x == 0) {
-        a.partial[2 * blockIdx.x] = s1;
-        a.partial[2 * blockIdx.x + 1] = s2;
-      }
-    }
     return;
   }
   // training: mean-square loss, dL/dy and dL/dh3
@@ -198,12 +194,69 @@ __global__ __launch_bounds__(256) void out_kernel(MpnnNet net, MpnnOutArgs a, co
     for (int o = 0; o < kHid; ++o) {
       float acc = 0.f;
 #pragma unroll
-      for (int j = 0; j < Np; ++j) acc = fmaf(bf2f(Wo[j * kHid + o]), dy[j], acc);
+      for (int j = 0; j < Np; ++j) acc = fmaf(weight_at(Wo, j * kHid + o), dy[j], acc);
       a.dh3[(size_t)c * kHid + o] = acc;
     }
   }
   const double s = block_sum256(loss, red);
   if (threadIdx.x == 0) a.loss_partial[blockIdx.x] = s;
+}
+
+// Preconditioner output with 64 cells per workgroup (lane = cell) and the Np
+// output rows split over the 4 waves: four times the waves of one thread per
+// cell. The tile's h3 rows (8 KB) are read once, coalesced, into LDS (row
+// stride padded to 17 words against bank conflicts) and shared by the waves.
+// z = zb + s_z c_t (W_o h3 + b_o) with the same products as out_kernel;
+// partials (r.z, z.q) per 64-cell tile.
+template <int N>
+__global__ __launch_bounds__(256) void precond_out_kernel(MpnnNet net, MpnnOutArgs a, const double* cgscal) {
+  constexpr int Np = Tri<N>::Np;
+  constexpr int Q = kHid / 4;  // 64-bit words (4 bf16) per h3 row
+  __shared__ double red[4];
+  __shared__ uint64_t hs[kPrecondCells][Q + 1];
+  if (cgscal && cgscal[kDone] != 0.0) return;
+  const int K = net.K;
+  const int lane = threadIdx.x & 63;
+  const int wave = uniform(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * kPrecondCells;
+  const int c = c0 + lane;
+  const bool valid = c < K;
+  for (int i = threadIdx.x; i < kPrecondCells * Q; i += 256) {
+    const int cell = i / Q, q = i % Q;
+    hs[cell][q] = c0 + cell < K ? reinterpret_cast<const uint64_t*>(a.h3 + (size_t)(c0 + cell) * kHid)[q] : 0;
+  }
+  __syncthreads();
+  float h[kHid];
+#pragma unroll
+  for (int q4 = 0; q4 < Q; ++q4) {
+    const uint64_t u = hs[lane][q4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) h[4 * q4 + v] = bf2f(uint16_t(u >> (16 * v)));
+  }
+  ConstU32* Wo = weight_pairs(net.wbf + net.L.oWo);
+  ConstF32* bo = weight_f32(net.wf + net.L.obo);
+  const double scale = a.dscal[kSz] * a.dscal[kCt];
+  double rz = 0.0, zq = 0.0;
+  for (int j = wave; j < Np; j += 4) {
+    float acc = bo[j];
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) acc = fmaf(weight_at(Wo, j * kHid + o), h[o], acc);
+    if (valid) {
+      const size_t idx = (size_t)j * K + c;
+      const double z = a.zb[idx] + scale * double(acc);
+      a.z[idx] = z;
+      if (a.r) rz += a.r[idx] * z;
+      if (a.q) zq += z * a.q[idx];
+    }
+  }
+  if (a.partial) {
+    const double s1 = block_sum256(rz, red);
+    const double s2 = block_sum256(zq, red);
+    if (threadIdx.x == 0) {
+      a.partial[2 * blockIdx.x] = s1;
+      a.partial[2 * blockIdx.x + 1] = s2;
+    }
+  }
 }
 
 // ---- energy-norm loss gradient ----------------------------------------------
@@ -222,7 +275,7 @@ __global__ __launch_bounds__(256) void energy_grad_kernel(MpnnNet net, const dou
   const int c = blockIdx.x * 256 + threadIdx.x;
   const bool valid = c < K;
   const double g = 2.0 * mscal[kSz] * mscal[kCt] / er;
-  const uint16_t* Wo = net.wbf + net.L.oWo;
+  ConstU32* Wo = weight_pairs(net.wbf + net.L.oWo);
   double zw = 0.0, zr = 0.0;
   float d[Np];
 #pragma unroll
@@ -242,7 +295,7 @@ __global__ __launch_bounds__(256) void energy_grad_kernel(MpnnNet net, const dou
     for (int o = 0; o < kHid; ++o) {
       float acc = 0.f;
 #pragma unroll
-      for (int j = 0; j < Np; ++j) acc = fmaf(bf2f(Wo[j * kHid + o]), d[j], acc);
+      for (int j = 0; j < Np; ++j) acc = fmaf(weight_at(Wo, j * kHid + o), d[j], acc);
       dh3[(size_t)c * kHid + o] = acc;
     }
   }
@@ -276,14 +329,14 @@ __global__ __launch_bounds__(256) void dense_bwd_kernel(MpnnNet net, int layer, 
     }
   }
   if (!u) return;
-  const uint16_t* A = net.wbf + net.L.oA[layer];
-  const uint16_t* B = net.wbf + net.L.oB[layer];
+  ConstU32* A = weight_pairs(net.wbf + net.L.oA[layer]);
+  ConstU32* B = weight_pairs(net.wbf + net.L.oB[layer]);
   for (int i = 0; i < D; ++i) {
     float su = 0.f, sv = 0.f;
 #pragma unroll
     for (int o = 0; o < kHid; ++o) {
-      su = fmaf(bf2f(A[o * D + i]), dz[o], su);
-      sv = fmaf(bf2f(B[o * D + i]), dz[o], sv);
+      su = fmaf(weight_at(A, o * D + i), dz[o], su);
+      sv = fmaf(weight_at(B, o * D + i), dz[o], sv);
     }
     u[(size_t)c * D + i] = su;
     v[(size_t)c * D + i] = sv;
@@ -452,12 +505,13 @@ hipError_t mpnn_dense_fwd(const MpnnNet& net, int layer, const uint16_t* hin, ui
 hipError_t mpnn_out(int order, const MpnnNet& net, int mode, const MpnnOutArgs& a, hipStream_t s,
                     const double* cgscal) {
   const int grid = (net.K + 255) / 256;
-#define CALL(n)                                                                                    \
-  if (mode == kOutY)                                                                               \
-    hipLaunchKernelGGL((out_kernel<n, kOutY>), dim3(grid), dim3(256), 0, s, net, a, cgscal);       \
-  else if (mode == kOutPrecond)                                                                    \
-    hipLaunchKernelGGL((out_kernel<n, kOutPrecond>), dim3(grid), dim3(256), 0, s, net, a, cgscal); \
-  else                                                                                             \
+  const int pgrid = (net.K + kPrecondCells - 1) / kPrecondCells;
+#define CALL(n)                                                                                      \
+  if (mode == kOutY)                                                                                 \
+    hipLaunchKernelGGL((out_kernel<n, kOutY>), dim3(grid), dim3(256), 0, s, net, a, cgscal);         \
+  else if (mode == kOutPrecond)                                                                      \
+    hipLaunchKernelGGL((precond_out_kernel<n>), dim3(pgrid), dim3(256), 0, s, net, a, cgscal);       \
+  else                                                                                               \
     hipLaunchKernelGGL((out_kernel<n, kOutTrain>), dim3(grid), dim3(256), 0, s, net, a, cgscal)
   DGC_N_SWITCH(order, CALL)
 #undef CALL

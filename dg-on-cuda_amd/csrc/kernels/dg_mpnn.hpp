@@ -15,6 +15,7 @@ namespace dgc {
 
 constexpr int kHid = 64;        // hidden width
 constexpr int kMpnnCells = 128; // cells per workgroup of the layer kernels (4 waves x 32)
+constexpr int kPrecondCells = 64;  // cells per workgroup of the preconditioner output
 
 struct MpnnLayout {
   int Np = 0, D0 = 0;  // D0 = 2 Np + 4 rounded up to a multiple of 8
@@ -68,7 +69,7 @@ enum MpnnOutMode { kOutY = 0, kOutPrecond = 1, kOutTrain = 2 };
 struct MpnnOutArgs {
   const uint16_t* h3 = nullptr;
   float* y = nullptr;           // kOutY: y [K][Np]
-  const double* zb = nullptr;   // kOutPrecond: z = zb + s_z c_t y (node-major)
+  const double* zb = nullptr;   // kOutPrecond: z = zb + s_z c_t y (node-major); partials per 64-cell tile
   double* z = nullptr;
   const double* r = nullptr;    // optional partial dots r.z and z.q
   const double* q = nullptr;

@@ -44,6 +44,7 @@ struct dgc_mpnn_s {
   uint64_t* wpk[3] = {nullptr, nullptr, nullptr};
   float* partial = nullptr;
   int pstride = 0, nblk = 0, out_grid = 0, cg_grid = 0;
+  int prec_grid = 0;  // workgroups (partial pairs) of the preconditioner output
   float* gnorm = nullptr;
   double* mscal = nullptr;
   double* skip_scal = nullptr;  // CG scalars with the done flag set (z = P r only)
@@ -281,6 +282,7 @@ int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, int base, dgc_mpnn_t* out) {
   m->L = dgc::mpnn_layout(m->Np);
   m->nblk = dgc::mpnn_grid(m->K);
   m->out_grid = (m->K + 255) / 256;
+  m->prec_grid = (m->K + dgc::kPrecondCells - 1) / dgc::kPrecondCells;
   m->cg_grid = s->grid;
   m->pstride = m->L.total;
   const size_t K = m->K, P = m->L.total;
@@ -299,7 +301,7 @@ int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, int base, dgc_mpnn_t* out) {
       (rc = dalloc(m, &m->dh, K * dgc::kHid)) || (rc = dalloc(m, &m->dh2, K * dgc::kHid)) ||
       (rc = dalloc(m, &m->u, K * dmax)) || (rc = dalloc(m, &m->vv, K * dmax)) || (rc = dalloc(m, &m->dy, K * m->Np)) ||
       (rc = dalloc(m, &m->y, K * m->Np)) || (rc = dalloc(m, &m->tgt, K * m->Np)) ||
-      (rc = dalloc(m, &m->loss_partial, m->out_grid)) || (rc = dalloc(m, &m->partial_out, 2 * m->out_grid)) ||
+      (rc = dalloc(m, &m->loss_partial, m->out_grid)) || (rc = dalloc(m, &m->partial_out, 2 * std::max(m->out_grid, m->prec_grid))) ||
       (rc = dalloc(m, &m->zfull, K * m->Np)) || (rc = dalloc(m, &m->amp, 32 * 32)) ||
       (rc = dalloc(m, &m->wbuf, K * m->Np)) || (rc = dalloc(m, &m->dotpart, 257)))
     return fail(rc);
@@ -597,7 +599,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   if (rc) return rc;
   DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, Lv.scal, m->mscal, nd, s->hist, st));
   if ((rc = precondition(m, Lv.r, Lv.z, nullptr, Lv.scal))) return rc;
-  DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->out_grid, 2, Lv.scal, m->mscal, nd, nullptr, st));
+  DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->prec_grid, 2, Lv.scal, m->mscal, nd, nullptr, st));
   DGC_LAUNCH(dgc::cg_direction_launch<double>(m->zfull, Lv.p, Lv.scal, n, st));
 
   // ---- one iteration, captured in chunks
@@ -629,7 +631,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 0, Lv.scal, m->mscal, nd, s->hist, st));
     int r2 = precondition(m, Lv.r, Lv.z, Lv.q, Lv.scal);
     if (r2) return r2;
-    DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->out_grid, 1, Lv.scal, m->mscal, nd, nullptr, st));
+    DGC_LAUNCH(dgc::fcg_scalars(m->partial_out, m->prec_grid, 1, Lv.scal, m->mscal, nd, nullptr, st));
     DGC_LAUNCH(dgc::cg_direction_launch<double>(m->zfull, Lv.p, Lv.scal, n, st));
     return DGC_OK;
   };
