@@ -333,29 +333,28 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
   __syncthreads();
 
   const int j0 = wave * RPW;
-  T ax[RPW], bx[RPW], ay[RPW], by[RPW];
+  // SPD form: the reference mass matrix is folded into the tables
+  // (M Dr, M Ds, M LIFT), so -J M Lap u needs no second product
+  const T* tDr = MODE == kSpd ? p.MDrT : p.DrT;
+  const T* tDs = MODE == kSpd ? p.MDsT : p.DsT;
+  const T* tL = MODE == kSpd ? p.MLT : p.LT;
+  // div q = Dr (rx qx + ry qy) + Ds (sx qx + sy qy): one accumulator per row
+  T lap[RPW];
 #pragma unroll
-  for (int i = 0; i < RPW; ++i) ax[i] = bx[i] = ay[i] = by[i] = 0;
+  for (int i = 0; i < RPW; ++i) lap[i] = 0;
 #pragma unroll
   for (int k = 0; k < Np; ++k) {
     const T qk = qxs[k][lane], rk = qys[k][lane];
-    const auto* a = table(p.DrT) + k * Npad + j0;
-    const auto* b = table(p.DsT) + k * Npad + j0;
+    const T gr = rx * qk + ry * rk, gs = sx * qk + sy * rk;
+    const auto* a = table(tDr) + k * Npad + j0;
+    const auto* b = table(tDs) + k * Npad + j0;
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      ax[i] = fma(a[i], qk, ax[i]);
-      bx[i] = fma(b[i], qk, bx[i]);
-      ay[i] = fma(a[i], rk, ay[i]);
-      by[i] = fma(b[i], rk, by[i]);
-    }
+    for (int i = 0; i < RPW; ++i) lap[i] = fma(b[i], gs, fma(a[i], gr, lap[i]));
   }
-  T lap[RPW];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) lap[i] = (rx * ax[i] + sx * bx[i]) + (ry * ay[i] + sy * by[i]);
 #pragma unroll
   for (int m = 0; m < NF; ++m) {
     const T f = lf[m][lane];
-    const auto* l = table(p.LT) + m * Npad + j0;
+    const auto* l = table(tL) + m * Npad + j0;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) lap[i] = fma(-l[i], f, lap[i]);
   }
@@ -370,31 +369,14 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
     }
     return;
   }
-  // SPD form: out = -J M lap. Rows of lap go through LDS (reusing qxs).
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    const int j = j0 + i;
-    if (j < Np) qxs[j][lane] = lap[i];
-  }
-  __syncthreads();
-  T o[RPW];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) o[i] = 0;
-#pragma unroll
-  for (int k = 0; k < Np; ++k) {
-    const T v = qxs[k][lane];
-    const auto* mm = table(p.MT) + k * Npad + j0;
-#pragma unroll
-    for (int i = 0; i < RPW; ++i) o[i] = fma(mm[i], v, o[i]);
-  }
+  // SPD form: out = -J (M Lap u), M already in the tables
   T dot = 0;
   if (valid) {
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const int j = j0 + i;
       if (j < Np) {
-        const T val = -Jc * o[i];
+        const T val = -Jc * lap[i];
         p.out[(size_t)j * K + c] = val;
         dot += dv[i] * val;
       }

@@ -40,6 +40,9 @@ struct IpdgParams {
   const T* __restrict__ DsT;
   const T* __restrict__ LT;   // [3Nfp][Npad]
   const T* __restrict__ MT;   // reference mass, [Np][Npad]
+  const T* __restrict__ MDrT;  // M Dr, M Ds, M LIFT (the SPD form's tables), same layouts
+  const T* __restrict__ MDsT;
+  const T* __restrict__ MLT;
   const T* __restrict__ geo;  // [kGeoComps][K]
   const T* __restrict__ J;    // [K]
   const T* __restrict__ tau;  // [3][K] penalty per face

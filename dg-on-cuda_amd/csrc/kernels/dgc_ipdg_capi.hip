@@ -104,16 +104,16 @@ unsigned grid_for(size_t n) { return unsigned(std::min<size_t>((n + 255) / 256, 
 
 // ---------------------------------------------------------------------------
 template <typename T>
-int IpdgLevel<T>::init(const dgc_ipdg_s& s, const std::vector<double>& drT, const std::vector<double>& dsT,
-                       const std::vector<double>& lT, const std::vector<double>& mT,
+int IpdgLevel<T>::init(const dgc_ipdg_s& s, const std::vector<std::vector<double>>& tables,
                        const std::vector<double>& geo, const std::vector<double>& J,
                        const std::vector<double>& tau) {
   const size_t n = size_t(s.disc->K) * s.disc->Np;
   const size_t nf = size_t(s.disc->K) * 3 * s.disc->Nfp;
   int rc;
-  if ((rc = upload(&DrT, cast<T>(drT))) || (rc = upload(&DsT, cast<T>(dsT))) || (rc = upload(&LT, cast<T>(lT))) ||
-      (rc = upload(&MT, cast<T>(mT))) || (rc = upload(&tau_d, cast<T>(tau))))
-    return rc;
+  T** dst[7] = {&DrT, &DsT, &LT, &MT, &MDrT, &MDsT, &MLT};
+  for (int i = 0; i < 7; ++i)
+    if ((rc = upload(dst[i], cast<T>(tables[i])))) return rc;
+  if ((rc = upload(&tau_d, cast<T>(tau)))) return rc;
   if (std::is_same<T, double>::value) {
     geo_d = reinterpret_cast<T*>(s.disc->geo);
     J_d = reinterpret_cast<T*>(s.disc->J);
@@ -136,6 +136,9 @@ void IpdgLevel<T>::release() {
   (void)hipFree(DsT);
   (void)hipFree(LT);
   (void)hipFree(MT);
+  (void)hipFree(MDrT);
+  (void)hipFree(MDsT);
+  (void)hipFree(MLT);
   (void)hipFree(tau_d);
   if (own_geo) {
     (void)hipFree(geo_d);
@@ -167,6 +170,9 @@ dgc::IpdgParams<T> IpdgLevel<T>::params(const dgc_ipdg_s& s) const {
   pr.DsT = DsT;
   pr.LT = LT;
   pr.MT = MT;
+  pr.MDrT = MDrT;
+  pr.MDsT = MDsT;
+  pr.MLT = MLT;
   pr.geo = geo_d;
   pr.J = J_d;
   pr.tau = tau_d;
@@ -434,10 +440,21 @@ int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const d
   int rc = DGC_OK;
   if (hipMemcpy(geo.data(), disc->geo, geo.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     rc = dgc_set_error(DGC_ERR_HIP, "dgc_ipdg_create: geometry readback failed");
-  const auto drT = transpose_pad(Dr, Np, Np), dsT = transpose_pad(Ds, Np, Np);
-  const auto lT = transpose_pad(lift, Np, NF), mT = transpose_pad(mass, Np, Np);
-  if (rc == DGC_OK) rc = s->f64.init(*s, drT, dsT, lT, mT, geo, J, tau);
-  if (rc == DGC_OK) rc = s->f32.init(*s, drT, dsT, lT, mT, geo, J, tau);
+  // the SPD form's tables M Dr, M Ds, M LIFT (row-major products)
+  auto mass_times = [&](const double* A, int cols) {
+    std::vector<double> P(size_t(Np) * cols, 0.0);
+    for (int i = 0; i < Np; ++i)
+      for (int k = 0; k < Np; ++k)
+        for (int j = 0; j < cols; ++j) P[size_t(i) * cols + j] += mass[size_t(i) * Np + k] * A[size_t(k) * cols + j];
+    return P;
+  };
+  const auto MDr = mass_times(Dr, Np), MDs = mass_times(Ds, Np), ML = mass_times(lift, NF);
+  const std::vector<std::vector<double>> tables = {
+      transpose_pad(Dr, Np, Np),         transpose_pad(Ds, Np, Np),         transpose_pad(lift, Np, NF),
+      transpose_pad(mass, Np, Np),       transpose_pad(MDr.data(), Np, Np), transpose_pad(MDs.data(), Np, Np),
+      transpose_pad(ML.data(), Np, NF)};
+  if (rc == DGC_OK) rc = s->f64.init(*s, tables, geo, J, tau);
+  if (rc == DGC_OK) rc = s->f32.init(*s, tables, geo, J, tau);
   if (rc == DGC_OK) rc = alloc(&s->hist, 1);
   if (rc == DGC_OK) rc = alloc(&s->dot, 257);
   if (rc != DGC_OK) {

@@ -16,6 +16,7 @@ struct IpdgLevel {
   T* DsT = nullptr;
   T* LT = nullptr;
   T* MT = nullptr;
+  T *MDrT = nullptr, *MDsT = nullptr, *MLT = nullptr;  // M Dr, M Ds, M LIFT
   T* tau_d = nullptr;
   T* geo_d = nullptr;  // FP64 level aliases the discretization's arrays
   T* J_d = nullptr;
@@ -35,8 +36,8 @@ struct IpdgLevel {
   bool graph_P = false;
   bool graph_coarse = false;
 
-  int init(const dgc_ipdg_s& s, const std::vector<double>& drT, const std::vector<double>& dsT,
-           const std::vector<double>& lT, const std::vector<double>& mT, const std::vector<double>& geo,
+  // tables: DrT, DsT, LT, MT, MDrT, MDsT, MLT (transposed and padded)
+  int init(const dgc_ipdg_s& s, const std::vector<std::vector<double>>& tables, const std::vector<double>& geo,
            const std::vector<double>& J, const std::vector<double>& tau);
   void release();
   dgc::IpdgParams<T> params(const dgc_ipdg_s& s) const;
