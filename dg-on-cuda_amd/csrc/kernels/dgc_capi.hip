@@ -651,59 +651,76 @@ int dgc_op_set_cache_policy(dgc_op_t op, int policy) {
 
 int dgc_op_cache_streaming(dgc_op_t op) { return op ? op_streaming(op) : -1; }
 
-// Times the stage-2 kernel (the widest: 5 state passes) of every supported
-// tiling on the caller's buffers and keeps the fastest. Only work vectors are
-// written (stage 2 writes wk0 and wk2, which the next step's stage 1
-// re-initialises), so the state is left untouched.
+// Times every supported tiling on the caller's buffers and keeps the one with
+// the shortest RK4 step: the four stage kernels in sequence, as a step runs
+// them, except that stage 4 updates wk2 (free once stage 3 has read it) in
+// place of the state: same reads and writes. Only work vectors wk0..wk3 are
+// written, which the next step's stage 1 re-initialises, so the state is left
+// untouched. ms_out[i] receives the mean stage-2 kernel time of candidate i
+// (timed on its own).
 int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
                     float* ms_out /* [4]: wt = 1, 2, 4, 0; -1 if not built */) {
   int rc = check_step_args(op, inout, size, wk, "dgc_op_autotune");
   if (rc != DGC_OK) return rc;
   DGC_REQUIRE(reps > 0, "dgc_op_autotune: reps must be positive");
   const int F = op->F;
-  dgc::StageParams p = base_params(op);
-  for (int f = 0; f < F; ++f) {
-    p.u[f] = inout[f];
-    p.acc[f] = wk[0 * F + f];
-    p.k3[f] = wk[3 * F + f];
-    p.in[f] = wk[1 * F + f];
-    p.out[f] = wk[2 * F + f];
+  // stage s reads in, writes out (and acc / k3 / u), as in one RK4 step
+  dgc::StageParams p[5];
+  for (int s = 1; s <= 4; ++s) {
+    p[s] = base_params(op);
+    p[s].c1 = 0.0;
+    for (int f = 0; f < F; ++f) {
+      p[s].u[f] = s == 4 ? wk[2 * F + f] : inout[f];
+      p[s].acc[f] = wk[0 * F + f];
+      p[s].k3[f] = wk[3 * F + f];
+      p[s].in[f] = s == 1 ? inout[f] : wk[(s == 3 ? 2 : 1) * F + f];
+      p[s].out[f] = wk[(s == 2 ? 2 : 1) * F + f];
+    }
   }
-  p.c1 = 0.0;
   hipStream_t st = op->disc->ctx->stream;
   const int cand[4] = {1, 2, 4, dgc::kMfmaTiling};
   float best = 0.f;
   int best_wt = op->wt;
-  hipEvent_t e0, e1;
+  hipEvent_t e0, e1, e2;
   DGC_HIP_TRY(hipEventCreate(&e0));
   DGC_HIP_TRY(hipEventCreate(&e1));
+  DGC_HIP_TRY(hipEventCreate(&e2));
+  const int order = op->disc->order, phys = op->physics;
   for (int i = 0; i < 4; ++i) {
     const int wt = cand[i];
     if (ms_out) ms_out[i] = -1.f;
-    if (dgc::supported_waves_per_tile(op->disc->order, op->physics, wt) != wt) continue;
+    if (dgc::supported_waves_per_tile(order, phys, wt) != wt) continue;
     if (op->precision == 1 && wt != 4) continue;  // the matrix-core kernel has one tiling
-    // warm-up launch, then timed repetitions
-    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, 2, wt, p, st);
+    // warm-up step, then reps launches of stage 2, then reps steps
+    hipError_t e = hipSuccess;
+    for (int s = 1; s <= 4 && e == hipSuccess; ++s) e = dgc::launch_stage(order, phys, s, wt, p[s], st);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = dgc::launch_stage(op->disc->order, op->physics, 2, wt, p, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = dgc::launch_stage(order, phys, 2, wt, p[2], st);
     if (e == hipSuccess) e = hipEventRecord(e1, st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+      for (int s = 1; s <= 4 && e == hipSuccess; ++s) e = dgc::launch_stage(order, phys, s, wt, p[s], st);
+    if (e == hipSuccess) e = hipEventRecord(e2, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e2);
+    float ms2 = 0.f, ms_step = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms2, e0, e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms_step, e1, e2);
     if (e != hipSuccess) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
       return dgc_set_error(DGC_ERR_LAUNCH, std::string("dgc_op_autotune: ") + hipGetErrorString(e));
     }
-    ms /= reps;
-    if (ms_out) ms_out[i] = ms;
-    if (best == 0.f || ms < best) {
-      best = ms;
+    ms2 /= reps;
+    if (ms_out) ms_out[i] = ms2;
+    const float step = ms_step / reps;
+    if (best == 0.f || step < best) {
+      best = step;
       best_wt = wt;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
   return dgc_op_set_waves_per_tile(op, best_wt);
 }
 
