@@ -88,6 +88,16 @@ int dgc_mesh_read_gmsh(const char* path, dgc_mesh_t* out) {
   });
 }
 
+int dgc_mesh_write_gmsh(dgc_mesh_t mesh, const char* path) {
+  return guarded("dgc_mesh_write_gmsh", [&]() -> int {
+    if (!mesh || !path) return dgc_set_error(DGC_ERR_INVALID, "dgc_mesh_write_gmsh: null argument");
+    std::string err;
+    if (!dgc_host::write_gmsh41(mesh->mesh, path, err))
+      return dgc_set_error(DGC_ERR_INVALID, "dgc_mesh_write_gmsh: " + err);
+    return DGC_OK;
+  });
+}
+
 int dgc_mesh_destroy(dgc_mesh_t mesh) {
   delete mesh;
   return DGC_OK;

@@ -233,6 +233,37 @@ void dof_coordinates(const Mesh2D& m, const std::vector<double>& r, const std::v
     }
 }
 
+bool write_gmsh41(const Mesh2D& m, const std::string& path, std::string& err) {
+  if (m.periodic) {
+    err = "a periodic mesh has no MSH 4.1 representation";
+    return false;
+  }
+  std::ofstream out(path);
+  if (!out) {
+    err = "cannot open " + path;
+    return false;
+  }
+  out.precision(17);
+  out << "$MeshFormat\n4.1 0 8\n$EndMeshFormat\n";
+  // nodes: numEntityBlocks numNodes minNodeTag maxNodeTag, then one block
+  // (entityDim 2, entityTag 1, parametric 0) with its tags and coordinates
+  out << "$Nodes\n1 " << m.Nv << " 1 " << m.Nv << "\n2 1 0 " << m.Nv << "\n";
+  for (int v = 1; v <= m.Nv; ++v) out << v << "\n";
+  for (int v = 0; v < m.Nv; ++v) out << m.px[size_t(v)] << " " << m.py[size_t(v)] << " 0\n";
+  out << "$EndNodes\n";
+  // elements: one block of 3-node triangles (element type 2)
+  out << "$Elements\n1 " << m.K << " 1 " << m.K << "\n2 1 2 " << m.K << "\n";
+  for (int c = 0; c < m.K; ++c)
+    out << c + 1 << " " << m.etov[3 * size_t(c)] + 1 << " " << m.etov[3 * size_t(c) + 1] + 1 << " "
+        << m.etov[3 * size_t(c) + 2] + 1 << "\n";
+  out << "$EndElements\n";
+  if (!out) {
+    err = "write failed: " + path;
+    return false;
+  }
+  return true;
+}
+
 bool read_gmsh41(const std::string& path, std::vector<double>& vx, std::vector<double>& vy,
                  std::vector<int>& etov, std::string& err) {
   std::ifstream in(path);

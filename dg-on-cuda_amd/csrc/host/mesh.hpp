@@ -78,6 +78,11 @@ void dof_coordinates(const Mesh2D& m, const std::vector<double>& r, const std::v
 
 // Minimal reader for gmsh MSH 4.1 ASCII files holding a 2D triangle mesh
 // (examples/square_domain_*.msh), returning vertices and triangle connectivity.
+// Writer of the same format: one node block and one triangle block (tags
+// 1..Nv and 1..K, z = 0), full double precision. Periodic meshes are rejected
+// (the format has no vertex identification). Returns false with err set on
+// failure.
+bool write_gmsh41(const Mesh2D& m, const std::string& path, std::string& err);
 bool read_gmsh41(const std::string& path, std::vector<double>& vx, std::vector<double>& vy,
                  std::vector<int>& etov, std::string& err);
 

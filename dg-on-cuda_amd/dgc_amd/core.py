@@ -48,6 +48,7 @@ def _declare(L):
                               C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp)])
     d("dgc_mesh_from_arrays", [C.c_int, _dp, _dp, C.c_int, _ip, C.POINTER(_vp)])
     d("dgc_mesh_read_gmsh", [C.c_char_p, C.POINTER(_vp)])
+    d("dgc_mesh_write_gmsh", [_vp, C.c_char_p])
     d("dgc_mesh_destroy", [_vp])
     d("dgc_mesh_info", [_vp, _ip, _ip, _dp])
     d("dgc_mesh_cells", [_vp, _dp, _dp, _ip, _ip, _ip])
@@ -181,6 +182,10 @@ class Mesh:
         h = _vp()
         check(lib().dgc_mesh_read_gmsh(str(path).encode(), C.byref(h)))
         return cls(h.value)
+
+    def write_gmsh(self, path):
+        """gmsh MSH 4.1 ASCII (one node block, one triangle block)."""
+        check(lib().dgc_mesh_write_gmsh(self._h, str(path).encode()))
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:

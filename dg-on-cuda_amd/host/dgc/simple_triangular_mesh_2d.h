@@ -67,6 +67,11 @@ class simple_triangular_mesh_2d {
     return m;
   }
 
+  // gmsh MSH 4.1 export (src/mesh/gmsh_importer_exporter.h:175-213).
+  void export_gmsh(const std::string& path) const {
+    check(dgc_mesh_write_gmsh(m_handle.get(), path.c_str()), "gmsh export failed");
+  }
+
   IT num_cells() const { return IT(m_nb_cell.size() / 3); }
   IT num_vertices() const { return IT(m_nv); }
   double min_edge_length() const { return m_hmin; }

@@ -76,6 +76,9 @@ int dgc_mesh_from_arrays(int nv, const double* vx, const double* vy, int k, cons
                          dgc_mesh_t* out);
 /* gmsh MSH 4.1 triangle mesh (src/mesh/gmsh_importer_exporter.h:49-173). */
 int dgc_mesh_read_gmsh(const char* path, dgc_mesh_t* out);
+/* Write a mesh as gmsh MSH 4.1 ASCII, one node and one triangle block
+ * (src/mesh/gmsh_importer_exporter.h:175-213); periodic meshes are rejected. */
+int dgc_mesh_write_gmsh(dgc_mesh_t mesh, const char* path);
 int dgc_mesh_destroy(dgc_mesh_t mesh);
 int dgc_mesh_info(dgc_mesh_t mesh, int* num_cells, int* num_vertices, double* min_edge);
 /* Per-cell vertex coordinates cx, cy [3K], vertex ids etov [3K] and the twin
