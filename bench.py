@@ -43,8 +43,8 @@ def stage_bytes(K, Np, F, stage):
     return K * (STAGE_PASSES[stage] * 8 * F * Np + GEO_BYTES_PER_CELL)
 
 
-def setup_advection(dgc, ctx, mesh, N, a=(1.0, 0.5)):
-    disc = dgc.Discretization(ctx, mesh, N)
+def setup_advection(dgc, ctx, mesh, N, a=(1.0, 0.5), setup="host"):
+    disc = dgc.Discretization(ctx, mesh, N, setup=setup)
     op = dgc.Advection(disc, a[0], a[1], dgc.Advection.BC_ZERO)
     x, y = mesh.dof_coords(N, node_major=True)
     u0 = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
@@ -54,9 +54,9 @@ def setup_advection(dgc, ctx, mesh, N, a=(1.0, 0.5)):
     return disc, op, state, work, dt, u0
 
 
-def setup_maxwell(dgc, ctx, mesh, N):
+def setup_maxwell(dgc, ctx, mesh, N, setup="host"):
     """Maxwell TM, Ez = sin(pi x) sin(pi y), Hx = Hy = 0, dt = 0.5 h_min / (N+1)^2."""
-    disc = dgc.Discretization(ctx, mesh, N)
+    disc = dgc.Discretization(ctx, mesh, N, setup=setup)
     op = dgc.MaxwellTM(disc)
     x, y = mesh.dof_coords(N, node_major=True)
     ez = np.sin(np.pi * x) * np.sin(np.pi * y)
@@ -68,9 +68,10 @@ def setup_maxwell(dgc, ctx, mesh, N):
     return disc, op, state, work, dt, ez
 
 
-def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64", physics="advection"):
-    setup = setup_maxwell if physics == "maxwell" else setup_advection
-    disc, op, state, work, dt, u0 = setup(dgc, ctx, mesh, N)
+def run_order(dgc, ctx, mesh, N, steps, warmup, prof_steps=5, precision="fp64", physics="advection",
+              setup="host"):
+    make = setup_maxwell if physics == "maxwell" else setup_advection
+    disc, op, state, work, dt, u0 = make(dgc, ctx, mesh, N, setup=setup)
     K, Np, F = disc.K, disc.Np, op.F
     op.precision = precision
     tune = op.autotune(state, work)  # picks the tiling (waves per 64-cell tile)
@@ -502,6 +503,26 @@ def maxwell_fp32_form(dgc, ctx, steps=50, warmup=5):
     return out
 
 
+def setup_timing(dgc, ctx, mesh, N):
+    """Wall time of building the device discretization of `mesh` at order N:
+    host set-up (geometry and connectivity in C++, then upload) against
+    device set-up (upload of the cell corners, geometry, half-edge sort and
+    packing on the GPU, host copies back). Both give the same arrays bit for
+    bit (tests/test_gpu_setup.py)."""
+    ctx.sync()
+    t0 = time.perf_counter()
+    d = dgc.Discretization(ctx, mesh, N, setup="host")
+    ctx.sync()
+    th = time.perf_counter() - t0
+    del d
+    t0 = time.perf_counter()
+    d = dgc.Discretization(ctx, mesh, N, setup="device")
+    ctx.sync()
+    td = time.perf_counter() - t0
+    del d
+    return dict(K=mesh.K, N=N, host_s=round(th, 3), device_s=round(td, 3), speedup=round(th / td, 1))
+
+
 def hbm_resident(dgc, ctx, steps, warmup):
     """Low-order FP64 RK4 with every vector in HBM: 2048 x 2048 periodic
     squares (K = 8,388,608, jitter 0.05h, seed 1), advection (a = (1, 0.5),
@@ -514,7 +535,7 @@ def hbm_resident(dgc, ctx, steps, warmup):
     out = []
     for physics in ("advection", "maxwell"):
         for N in (1, 2, 3):
-            r = run_order(dgc, ctx, hm, N, steps, warmup, prof_steps=3, physics=physics)
+            r = run_order(dgc, ctx, hm, N, steps, warmup, prof_steps=3, physics=physics, setup="device")
             roof = roofline_for(r)
             out.append(dict(physics=physics, N=N, K=r["K"], value=r["value"], finite=r["finite"],
                             stage_us=[round(v * 1e3, 1) for v in r["stage_ms"]],
@@ -525,7 +546,9 @@ def hbm_resident(dgc, ctx, steps, warmup):
                   f"stage-2 {roof['achieved']} GB/s ({roof['frac']:.1%}), whole step {out[-1]['step_GBs']} GB/s",
                   file=sys.stderr, flush=True)
             del r
-    return out
+    st = setup_timing(dgc, ctx, hm, 3)
+    print(f"# set-up K={st['K']} N=3: host {st['host_s']} s, device {st['device_s']} s", file=sys.stderr, flush=True)
+    return out, st
 
 
 def main():
@@ -574,7 +597,7 @@ def main():
         return
     if args.hbm_only:
         hm = dgc.Mesh.structured(2048, 2048, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
-        r = run_order(dgc, ctx, hm, 3, args.steps, args.warmup, physics=args.hbm_physics)
+        r = run_order(dgc, ctx, hm, 3, args.steps, args.warmup, physics=args.hbm_physics, setup="device")
         print(json.dumps(dict(workload=f"{args.hbm_physics} fp64 rk4 K={r['K']} N=3 (HBM-resident)",
                               value=r["value"], roofline=roofline_for(r),
                               stage_us=[round(v * 1e3, 2) for v in r["stage_ms"]])))
@@ -610,7 +633,7 @@ def main():
 
     extra = {}
     if not args.no_hbm:
-        extra["hbm_resident"] = hbm_resident(dgc, ctx, max(10, args.steps // 20), 2)
+        extra["hbm_resident"], extra["setup"] = hbm_resident(dgc, ctx, max(10, args.steps // 20), 2)
     if not args.no_fp32:
         extra["maxwell_fp32_mfma"] = maxwell_fp32_form(dgc, ctx)
 

@@ -57,6 +57,8 @@ def _declare(L):
     d("dgc_disc2d_create", [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _ip, _ip, C.c_int,
                             _dp, _dp, _dp, _dp, C.POINTER(_vp)])
     d("dgc_disc2d_destroy", [_vp])
+    d("dgc_disc2d_create_from_cells", [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.POINTER(_vp)])
+    d("dgc_disc2d_arrays", [_vp, _dp, _ip, _dp, _dp, _ip, _dp, _dp])
     d("dgc_maxwell_tm_create", [_vp, _dp, _dp, _dp, C.POINTER(_vp)])
     d("dgc_advection_create", [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)])
     d("dgc_op_destroy", [_vp])
@@ -318,17 +320,40 @@ class DeviceArray:
 
 
 class Discretization:
-    """Device copy of the geometry of `mesh` at order `order`."""
+    """Device copy of the geometry of `mesh` at order `order`.
 
-    def __init__(self, ctx: Context, mesh: Mesh, order: int, geometry: dict | None = None):
+    setup="host" computes the geometric factors and connectivity on the host
+    (C++) and uploads them; setup="device" uploads only the cell corners and
+    vertex ids and builds the same arrays, bit for bit, on the GPU (the
+    choice for meshes of millions of cells)."""
+
+    SETUPS = ("host", "device")
+
+    def __init__(self, ctx: Context, mesh: Mesh, order: int, geometry: dict | None = None,
+                 setup: str = "host"):
+        if setup not in self.SETUPS:
+            raise ValueError(f"setup must be one of {self.SETUPS}")
         self.ctx = ctx
         self.mesh = mesh
         self.order = order
         self.K = mesh.K
         self.Np = (order + 1) * (order + 2) // 2
         self.Nfp = order + 1
+        self.setup = setup
+        if setup == "device":
+            if geometry is not None:
+                raise ValueError("setup='device' builds the geometry itself")
+            n = 3 * self.K
+            cx, cy, ev = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
+            check(lib().dgc_mesh_cells(mesh._h, _d(cx), _d(cy), _i(ev), None, None))
+            h = _vp()
+            check(lib().dgc_disc2d_create_from_cells(ctx.handle, self.K, mesh.Nv, order, _d(cx), _d(cy), _i(ev),
+                                                     C.byref(h)))
+            self._h = h
+            self._geometry = None
+            return
         g = geometry if geometry is not None else mesh.geometry(order)
-        self.geometry = g
+        self._geometry = g
         ref = ref_operators(order, "weak")
         fm = ref["fmask"]
         f0 = np.ascontiguousarray(fm[0], np.int32)
@@ -343,6 +368,30 @@ class Discretization:
                                       g["num_boundary_nodes"], _d(bx), _d(by), _d(g["nx"]),
                                       _d(g["ny"]), C.byref(h)))
         self._h = h
+
+    @property
+    def geometry(self) -> dict:
+        """Host geometry arrays (dgc_mesh_geometry layout); computed on first
+        use for a device set-up."""
+        if self._geometry is None:
+            self._geometry = self.mesh.geometry(self.order)
+        return self._geometry
+
+    def arrays(self) -> dict:
+        """Host copies of the device arrays: geo [13, K], nbr [3, K], J [K],
+        face_J [3, K], bx, by."""
+        K = self.K
+        geo = np.zeros((13, K))
+        nbr = np.zeros((3, K), np.int32)
+        J = np.zeros(K)
+        fJ = np.zeros((3, K))
+        nb = C.c_int()
+        check(lib().dgc_disc2d_arrays(self._h, _d(geo), _i(nbr), _d(J), _d(fJ), C.byref(nb), None, None))
+        bx = np.zeros(max(nb.value, 1))
+        by = np.zeros(max(nb.value, 1))
+        check(lib().dgc_disc2d_arrays(self._h, None, None, None, None, None, _d(bx), _d(by)))
+        return dict(geo=geo, nbr=nbr, J=J, face_J=fJ, bx=bx[:nb.value], by=by[:nb.value],
+                    num_boundary_nodes=nb.value)
 
     @property
     def handle(self):

@@ -2,8 +2,10 @@
 // the flow of examples/maxwell_2d/main.cpp:65-231: mesh -> problem ->
 // fill_* -> device object -> time loop of rk4_on_device -> error norm.
 //
-//   maxwell_2d [order] [mesh.msh | structured]   (default: order 4, the
-//   16x16 jittered cavity mesh, dt = 0.5 h_min / (N+1)^2, T = 1)
+//   maxwell_2d [order] [mesh.msh | structured] [host | device]
+//   (default: order 4, the 16x16 jittered cavity mesh, dt = 0.5 h_min / (N+1)^2,
+//   T = 1, geometry and connectivity set up on the host; "device" builds them
+//   on the GPU from the mesh cells)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +25,8 @@ int main(int argc, char** argv) {
   auto mesh = src == "structured" ? Mesh::structured(16, 16, -1, 1, -1, 1, 0.1, 0) : Mesh::import_gmsh(src);
 
   dgc::maxwell_2d<double, Mesh> op(*mesh, order);
-  auto disc = dgc::create_on_device(op);
+  const std::string setup = argc > 3 ? argv[3] : "host";
+  auto disc = setup == "device" ? dgc::create_on_device_from_mesh(*mesh, order) : dgc::create_on_device(op);
   dgc::d_maxwell_2d* dOp = dgc::create_device_object(disc.get(), op.m_Dr.data(), op.m_Ds.data(), op.m_L.data());
 
   const int n = op.total_num_nodes();

@@ -127,6 +127,24 @@ device_discretization create_on_device(const simple_discretization_2d<T, M>& h, 
       fJ.data(), ic.data(), iff.data(), nb, bx.data(), by.data(), nx.data(), ny.data(), ctx);
 }
 
+// Device set-up from the mesh alone (dgc_disc2d_create_from_cells): geometry,
+// connectivity and boundary nodes built on the GPU from the cell corners and
+// vertex ids; the same arrays, bit for bit, as create_on_device, without the
+// host marshalling of src/simple_discretization_2d.h:84-218 (meshes of millions
+// of cells).
+template <typename M>
+device_discretization create_on_device_from_mesh(const M& mesh, int order, dgc_ctx_t ctx = default_context()) {
+  const std::size_t n = 3 * std::size_t(mesh.num_cells());
+  std::vector<double> cx(n), cy(n);
+  std::vector<int> etov(n);
+  check(dgc_mesh_cells(mesh.handle(), cx.data(), cy.data(), etov.data(), nullptr, nullptr), "mesh cells failed");
+  dgc_disc_t d = nullptr;
+  check(dgc_disc2d_create_from_cells(ctx, int(mesh.num_cells()), int(mesh.num_vertices()), order, cx.data(),
+                                     cy.data(), etov.data(), &d),
+        "device set-up failed");
+  return device_discretization(d, &dgc_disc2d_destroy);
+}
+
 END_NAMESPACE
 
 #endif

@@ -104,6 +104,30 @@ int dgc_disc2d_create(dgc_ctx_t ctx, int num_cells, int order, const int* face0_
                       const double* by, const double* nx, const double* ny, dgc_disc_t* out);
 int dgc_disc2d_destroy(dgc_disc_t disc);
 
+/* Device set-up: the same discretization built on the GPU from the cells
+ * alone, for meshes where the host set-up dominates (K >= 1e6). Replaces the
+ * host marshalling of src/simple_discretization_2d.h:84-218 (geometric factors,
+ * src/mapping_2d.h:101-158; outward normals, src/basic_geom_2d.h:110-153) and
+ * the twin half-edge search of src/mesh/simple_triangular_mesh_2d.h:151-217.
+ * cx, cy: [3 K] host arrays, the counter-clockwise corner coordinates of each
+ * cell (periodic meshes: unwrapped); etov: [3 K] host array of vertex ids in
+ * [0, num_vertices), equal ids meaning a shared vertex. Every device and host
+ * array of the result equals, bit for bit, what dgc_mesh_geometry +
+ * dgc_disc2d_create give for the same mesh (boundary nodes numbered in cell,
+ * then face order). Errors: DGC_ERR_INVALID for a vertex id out of range, a
+ * degenerate or non-manifold edge, or a clockwise / degenerate cell. */
+int dgc_disc2d_create_from_cells(dgc_ctx_t ctx, int num_cells, int num_vertices, int order,
+                                 const double* cx, const double* cy, const int* etov,
+                                 dgc_disc_t* out);
+/* Copies of a discretization's device arrays into host memory (any pointer may
+ * be NULL): geo [13][K] (rx, ry, sx, sy, face_J/J for faces 0-2, nx for faces
+ * 0-2, ny for faces 0-2), nbr [3][K] (twin cell << 2 | twin face, or boundary
+ * node offset << 2 | 3), J [K], face_J [3][K], and the boundary face nodes.
+ * No counterpart in the reference (its device copies are plain pointers the
+ * caller can read). */
+int dgc_disc2d_arrays(dgc_disc_t disc, double* geo, int* nbr, double* J, double* face_J,
+                      int* num_boundary_nodes, double* bx, double* by);
+
 /* ---- semi-discrete operators (device objects) ----------------------------
  * Replace create_device_object / destroy_device_object of
  * examples/maxwell_2d/k_maxwell_2d.cu:45-116 and

@@ -49,3 +49,6 @@ def test_null_and_range_checks_without_gpu():
     assert L.dgc_ref_operators(1, 5, *([None] * 7)) == 1
     assert L.dgc_ctx_sync(None) == 1
     assert L.dgc_axpy(None, 1.0, None, 0, None, None) == 1
+    assert L.dgc_disc2d_create_from_cells(None, 8, 9, 1, None, None, None, None) == 1
+    assert b"null" in L.dgc_last_error()
+    assert L.dgc_disc2d_arrays(None, None, None, None, None, None, None, None) == 1

@@ -27,6 +27,14 @@ def test_maxwell_driver_cavity_mode():
     assert float(m.group(2)) < 1e-8
 
 
+def test_maxwell_driver_with_device_setup():
+    # geometry and connectivity built on the GPU: the same run to the last digit
+    host = run("maxwell_2d", 4, "structured", "host")
+    dev = run("maxwell_2d", 4, "structured", "device")
+    pick = lambda out: re.search(r"steps = \d+.*error norm = [0-9.e+-]+", out).group(0)
+    assert pick(dev) == pick(host)
+
+
 def test_maxwell_driver_on_reference_gmsh_mesh():
     # K = 16 gmsh mesh of the reference examples; the reference CPU path gives
     # a mean-square error of about 1.7e-3 at N = 3, T = 1 on it
