@@ -483,6 +483,41 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     return out
 
 
+def graddiv_bench(dgc, ctx, N=3, reps=30):
+    """grad_2d and div_2d with central fluxes (dgc_grad2d_apply / dgc_div2d_apply)
+    on 1024 x 1024 squares (K = 2,097,152, jitter 0.05h, seed 1) at order N:
+    mean time per launch over `reps` back-to-back launches on the context's
+    stream (wall clock around them, launch overhead included). Algorithmic
+    HBM bytes per cell: grad reads u and writes two components, div reads two
+    and writes one, both 24 Np + 44 (8 B per node value, four metric terms,
+    three connectivity words)."""
+    mesh = dgc.Mesh.structured(1024, 1024, 0, 1, 0, 1, jitter=0.05, seed=1)
+    disc = dgc.Discretization(ctx, mesh, N, setup="device")
+    op = dgc.GradDiv(disc)
+    n = disc.size
+    x, y = mesh.dof_coords(N, node_major=True)
+    u0 = dgc.DeviceArray.from_host(ctx, np.sin(np.pi * x) * np.cos(np.pi * y))
+    u1 = dgc.DeviceArray.from_host(ctx, np.cos(np.pi * x) * np.sin(np.pi * y))
+    o0, o1 = dgc.DeviceArray(ctx, n), dgc.DeviceArray(ctx, n)
+    out = {}
+    bytes_per = (24 * disc.Np + 44) * disc.K
+    for name, call in (("grad", lambda: op.grad(u0, o0, o1)), ("div", lambda: op.div(u0, u1, o0))):
+        for _ in range(3):
+            call()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        ctx.sync()
+        dur = (time.perf_counter() - t0) / reps
+        gbs = bytes_per / dur / 1e9
+        out[name] = dict(us=round(dur * 1e6, 1), GBs=round(gbs, 1), frac_of_peak=round(gbs / HBM_PEAK_GBS, 4))
+    out.update(K=disc.K, N=N, bytes_per_launch=bytes_per)
+    print(f"# grad/div K={disc.K} N={N}: grad {out['grad']['us']} us ({out['grad']['GBs']} GB/s), "
+          f"div {out['div']['us']} us ({out['div']['GBs']} GB/s)", file=sys.stderr, flush=True)
+    return out
+
+
 def maxwell_fp32_form(dgc, ctx, steps=50, warmup=5):
     """Maxwell TM on the sweep's 256 x 256 periodic lattice (K = 131072),
     N = 4..7, Ez = sin(pi x) sin(pi y): the FP64 path (autotuned tiling)
@@ -570,6 +605,7 @@ def main():
     ap.add_argument("--no-ipdg", action="store_true", help="skip the IPDG solver comparison")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 matrix-core form of the sweep")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched Maxwell ensemble")
+    ap.add_argument("--no-graddiv", action="store_true", help="skip the gradient / divergence timing")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ipdg-full", action="store_true",
                     help="IPDG comparison at full size: 4096 training pairs, 256 held-out right-hand sides")
@@ -636,6 +672,8 @@ def main():
         extra["hbm_resident"], extra["setup"] = hbm_resident(dgc, ctx, max(10, args.steps // 20), 2)
     if not args.no_fp32:
         extra["maxwell_fp32_mfma"] = maxwell_fp32_form(dgc, ctx)
+    if not args.no_graddiv:
+        extra["graddiv"] = graddiv_bench(dgc, ctx)
 
     if not args.no_ipdg:
         full = args.ipdg_full

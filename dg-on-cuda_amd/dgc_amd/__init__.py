@@ -19,5 +19,6 @@ from .core import (  # noqa: F401
     cell_to_node_major,
 )
 from .ipdg import IPDG, poisson_rhs  # noqa: F401
+from .graddiv import GradDiv  # noqa: F401
 from .mpnn import LearnedPreconditioner  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401

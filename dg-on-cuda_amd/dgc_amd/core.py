@@ -59,6 +59,10 @@ def _declare(L):
     d("dgc_disc2d_destroy", [_vp])
     d("dgc_disc2d_create_from_cells", [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.POINTER(_vp)])
     d("dgc_disc2d_arrays", [_vp, _dp, _ip, _dp, _dp, _ip, _dp, _dp])
+    d("dgc_graddiv_create", [_vp, _dp, _dp, _dp, C.POINTER(_vp)])
+    d("dgc_graddiv_destroy", [_vp])
+    d("dgc_grad2d_apply", [_vp, _vp, _vp, C.c_double, _vp, _vp])
+    d("dgc_div2d_apply", [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
     d("dgc_maxwell_tm_create", [_vp, _dp, _dp, _dp, C.POINTER(_vp)])
     d("dgc_advection_create", [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)])
     d("dgc_op_destroy", [_vp])

@@ -37,6 +37,7 @@ enum dgc_status {
 
 typedef struct dgc_ctx_s* dgc_ctx_t;
 typedef struct dgc_disc_s* dgc_disc_t;
+typedef struct dgc_graddiv_s* dgc_graddiv_t;
 typedef struct dgc_op_s* dgc_op_t;
 typedef struct dgc_mesh_s* dgc_mesh_t;
 
@@ -127,6 +128,29 @@ int dgc_disc2d_create_from_cells(dgc_ctx_t ctx, int num_cells, int num_vertices,
  * caller can read). */
 int dgc_disc2d_arrays(dgc_disc_t disc, double* geo, int* nbr, double* J, double* face_J,
                       int* num_boundary_nodes, double* bx, double* by);
+
+/* ---- gradient and divergence with central fluxes -----------------------
+ * Replace grad_2d (src/grad_2d.h:36-187) and div_2d (src/div_2d.h:36-188),
+ * the operators either side of the Poisson solve in the Stokes splitting.
+ * Dr, Ds: weak-form derivative matrices (V Vr^T M), lift: LIFT, all
+ * row-major as dgc_ref_operators(order, 0, ...) returns them. The reference's
+ * BC object becomes its affine form: on boundary node b (numbered cell by
+ * cell, face by face, as dgc_disc2d_arrays' bx, by) the exterior value is
+ * g[b] + beta * (interior value); g == NULL means g = 0 (beta = 1: the
+ * "exterior = interior" BC of stokes_2d.h:162-163; g = 2 u_D, beta = -1: a
+ * Dirichlet value u_D). u, g, out: device arrays, node-major [Np][K]; out of
+ * place (in place in the reference: the caller copies first). Asynchronous
+ * on the context's stream. */
+int dgc_graddiv_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
+                       dgc_graddiv_t* out);
+int dgc_graddiv_destroy(dgc_graddiv_t op);
+/* (out_x, out_y) = grad u  (grad_2d::operator(), src/grad_2d.h:152-187) */
+int dgc_grad2d_apply(dgc_graddiv_t op, const double* u, const double* g, double beta, double* out_x,
+                     double* out_y);
+/* out = div (u0, u1)  (div_2d::operator(), src/div_2d.h:154-188); g0, g1: the
+ * boundary data of the two components */
+int dgc_div2d_apply(dgc_graddiv_t op, const double* u0, const double* u1, const double* g0,
+                    const double* g1, double beta, double* out);
 
 /* ---- semi-discrete operators (device objects) ----------------------------
  * Replace create_device_object / destroy_device_object of

@@ -366,6 +366,123 @@ void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, cons
   }
 }
 
+// Boundary-node offsets of the faces of boundary half-edges, counted cell by
+// cell, face by face (the order the reference calls its BC objects in); -1 for
+// interior faces.
+static std::vector<int> boundary_offsets(int K, int Nfp, const int* nb_cell) {
+  std::vector<int> off(size_t(3) * K, -1);
+  int b = 0;
+  for (int c = 0; c < K; ++c)
+    for (int e = 0; e < 3; ++e)
+      if (nb_cell[3 * c + e] == c) {
+        off[3 * c + e] = b;
+        b += Nfp;
+      }
+  return off;
+}
+
+// Weak gradient with central flux: src/grad_2d.h:95-187. The BC object is given
+// in its affine form: exterior value g[b] + beta * interior at boundary node b
+// (g == NULL: 0). Weak-form Dr, Ds. in: K*Np cell-major; out: ox, oy.
+void oracle_grad(int K, int N, const double* Dr, const double* Ds, const double* L, const double* cx,
+                 const double* cy, const int* nb_cell, const int* nb_face, const double* g, double beta,
+                 const double* in, double* ox, double* oy) {
+  const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  const std::vector<int> boff = boundary_offsets(K, Nfp, nb_cell);
+  std::vector<double> nf(size_t(K) * 3 * Nfp);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < K; ++c)
+    for (int e = 0; e < 3; ++e)
+      for (int d = 0; d < Nfp; ++d) {
+        const double aU = in[size_t(c) * Np + face_node(N, e, d)];
+        double bU;
+        if (nb_cell[3 * c + e] == c)
+          bU = (g ? g[boff[3 * c + e] + d] : 0.0) + beta * aU;
+        else
+          bU = in[size_t(nb_cell[3 * c + e]) * Np + face_node(N, nb_face[3 * c + e], Nfp - d - 1)];
+        nf[size_t(c) * 3 * Nfp + e * Nfp + d] = (aU + bU) / 2.0;
+      }
+#pragma omp parallel
+  {
+  std::vector<double> Dx(Np * Np), Dy(Np * Np), fx(3 * Nfp), fy(3 * Nfp);
+#pragma omp for schedule(static)
+  for (int c = 0; c < K; ++c) {
+    const Cell cell = get_cell(cx, cy, c);
+    double ij[4];
+    inv_jacobian(cell, ij);
+    combine(Dr, Ds, Np * Np, ij[0], ij[2], Dx.data());
+    combine(Dr, Ds, Np * Np, ij[1], ij[3], Dy.data());
+    const size_t o = size_t(c) * Np;
+    gemv(Dx.data(), Np, Np, -1.0, in + o, 0.0, ox + o);
+    gemv(Dy.data(), Np, Np, -1.0, in + o, 0.0, oy + o);
+    for (int e = 0; e < 3; ++e) {
+      const double fj = face_J(cell, e);
+      double nx, ny;
+      outward_normal(cell, e, nx, ny);
+      for (int i = 0; i < Nfp; ++i) {
+        fx[e * Nfp + i] = nf[size_t(c) * 3 * Nfp + e * Nfp + i] * nx * fj;
+        fy[e * Nfp + i] = nf[size_t(c) * 3 * Nfp + e * Nfp + i] * ny * fj;
+      }
+    }
+    gemv(L, Np, 3 * Nfp, 1.0 / cell_J(cell), fx.data(), 1.0, ox + o);
+    gemv(L, Np, 3 * Nfp, 1.0 / cell_J(cell), fy.data(), 1.0, oy + o);
+  }
+  }
+}
+
+// Weak divergence with central flux: src/div_2d.h:95-188, BC as oracle_grad
+// with boundary data g0, g1 for the two components. out: K*Np.
+void oracle_div(int K, int N, const double* Dr, const double* Ds, const double* L, const double* cx,
+                const double* cy, const int* nb_cell, const int* nb_face, const double* g0, const double* g1,
+                double beta, const double* u0, const double* u1, double* out) {
+  const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  const std::vector<int> boff = boundary_offsets(K, Nfp, nb_cell);
+  std::vector<double> nf(size_t(K) * 3 * Nfp);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < K; ++c) {
+    const Cell cell = get_cell(cx, cy, c);
+    for (int e = 0; e < 3; ++e) {
+      double nx, ny;
+      outward_normal(cell, e, nx, ny);
+      for (int d = 0; d < Nfp; ++d) {
+        const size_t ia = size_t(c) * Np + face_node(N, e, d);
+        const double a0 = u0[ia], a1 = u1[ia];
+        double b0, b1;
+        if (nb_cell[3 * c + e] == c) {
+          const int b = boff[3 * c + e] + d;
+          b0 = (g0 ? g0[b] : 0.0) + beta * a0;
+          b1 = (g1 ? g1[b] : 0.0) + beta * a1;
+        } else {
+          const size_t ib = size_t(nb_cell[3 * c + e]) * Np + face_node(N, nb_face[3 * c + e], Nfp - d - 1);
+          b0 = u0[ib];
+          b1 = u1[ib];
+        }
+        nf[size_t(c) * 3 * Nfp + e * Nfp + d] = ((a0 + b0) * nx + (a1 + b1) * ny) / 2.0;
+      }
+    }
+  }
+#pragma omp parallel
+  {
+  std::vector<double> Dx(Np * Np), Dy(Np * Np), fl(3 * Nfp);
+#pragma omp for schedule(static)
+  for (int c = 0; c < K; ++c) {
+    const Cell cell = get_cell(cx, cy, c);
+    double ij[4];
+    inv_jacobian(cell, ij);
+    combine(Dr, Ds, Np * Np, ij[0], ij[2], Dx.data());
+    combine(Dr, Ds, Np * Np, ij[1], ij[3], Dy.data());
+    const size_t o = size_t(c) * Np;
+    gemv(Dx.data(), Np, Np, -1.0, u0 + o, 0.0, out + o);
+    gemv(Dy.data(), Np, Np, -1.0, u1 + o, 1.0, out + o);
+    for (int e = 0; e < 3; ++e) {
+      const double fj = face_J(cell, e);
+      for (int i = 0; i < Nfp; ++i) fl[e * Nfp + i] = nf[size_t(c) * 3 * Nfp + e * Nfp + i] * fj;
+    }
+    gemv(L, Np, 3 * Nfp, 1.0 / cell_J(cell), fl.data(), 1.0, out + o);
+  }
+  }
+}
+
 // Classical RK4 of src/explicit_runge_kutta.h:33-51 with the host axpy_n of
 // src/axpy.h:112-122, for either physics (phys 0: Maxwell, 1: advection).
 // u: F*K*Np cell-major, advanced in place over `nsteps` steps of dt from t0;

@@ -25,6 +25,8 @@
 
 #include "advection_2d.h"
 #include "axpy.h"
+#include "div_2d.h"
+#include "grad_2d.h"
 #include "explicit_runge_kutta.h"
 #include "maxwell_2d.h"
 #include "poisson_2d.h"
@@ -296,6 +298,56 @@ int main(int argc, char** argv) {
       in.read((char*)v.data(), 8 * n);
       op(v.begin());
       write(f, v);
+    }
+    return 0;
+  }
+
+  if ((cmd == "grad" || cmd == "div") && argc == 7) {
+    // grad_2d::operator() (src/grad_2d.h:152-187) on each vector, or
+    // div_2d::operator() (src/div_2d.h:154-188) on each pair of vectors, of an
+    // input file of f64 (CPU layout, K*Np per vector). BC (exterior value at
+    // boundary node (x, y), interior value a): bc 0: a; bc 1: 2 sin(x) cos(y) - a
+    // (first component), 2 cos(x) sin(y) - a (second). args: mesh N bc infile out.
+    // out: grad: per vector the x then the y component; div: per pair the divergence.
+    Mesh m = load_mesh(argv[2]);
+    const int N = std::atoi(argv[3]);
+    const int bc = std::atoi(argv[4]);
+    struct BC1 {
+      int kind;
+      double exterior_val(double x, double y, double a) const { return kind ? 2.0 * std::sin(x) * std::cos(y) - a : a; }
+    };
+    struct BC2 {
+      int kind;
+      std::pair<double, double> exterior_val(double x, double y, std::pair<double, double> a) const {
+        if (!kind) return a;
+        return std::make_pair(2.0 * std::sin(x) * std::cos(y) - a.first, 2.0 * std::cos(x) * std::sin(y) - a.second);
+      }
+    };
+    const std::size_t n = std::size_t(m.num_cells()) * (N + 1) * (N + 2) / 2;
+    std::ifstream in(argv[5], std::ios::binary | std::ios::ate);
+    const std::size_t bytes = in.tellg();
+    in.seekg(0);
+    const std::size_t per = cmd == "grad" ? n : 2 * n;
+    if (bytes % (8 * per) != 0) die(cmd + ": input size is not a multiple of the vector size");
+    const std::size_t nvec = bytes / (8 * per);
+    std::ofstream f(argv[6], std::ios::binary);
+    std::vector<double> v0(n), v1(n);
+    if (cmd == "grad") {
+      dgc::grad_2d<double, Mesh> op(m, N);
+      for (std::size_t k = 0; k < nvec; ++k) {
+        in.read((char*)v0.data(), 8 * n);
+        op(v0.begin(), v1.begin(), BC1{bc});
+        write(f, v0);
+        write(f, v1);
+      }
+    } else {
+      dgc::div_2d<double, Mesh> op(m, N);
+      for (std::size_t k = 0; k < nvec; ++k) {
+        in.read((char*)v0.data(), 8 * n);
+        in.read((char*)v1.data(), 8 * n);
+        op(v0.begin(), v1.begin(), BC2{bc});
+        write(f, v0);
+      }
     }
     return 0;
   }

@@ -135,8 +135,40 @@ def main():
     np.savez_compressed(os.path.join(HERE, "laplace_ref.npz"), vx=vx, vy=vy, etov=etov, order=N,
                         vin=vin, vout=vout, coarse_vx=cvx, coarse_vy=cvy, coarse_etov=cetov,
                         coarse_order=2, coarse_A=A)
+    graddiv(tmp)
     print("golden vectors written to", HERE)
 
 
+def graddiv(tmp):
+    """5. grad_2d and div_2d (central fluxes, weak form) on the K = 16 gmsh mesh at
+    N = 3 and a jittered 8 x 6 mesh of [-1, 1]^2 at N = 2, two seeded vectors
+    (pairs for div), BC 0 (exterior = interior) and BC 1 (exterior =
+    2 sin(x) cos(y) - interior, second component 2 cos(x) sin(y) - interior)."""
+    out = {}
+    meshes = {"coarse": (dgc.Mesh.read_gmsh(os.path.join(HERE, "square_domain_coarse.msh")), 3),
+              "grid": (dgc.Mesh.structured(8, 6, -1, 1, -1, 1, jitter=0.15, seed=11), 2)}
+    for name, (m, N) in meshes.items():
+        vx, vy, etov = mesh_arrays(m)
+        mpath = os.path.join(tmp, f"gd_{name}.bin")
+        write_mesh(mpath, vx, vy, etov)
+        n = m.K * (N + 1) * (N + 2) // 2
+        rng = np.random.default_rng(13)
+        vin = rng.standard_normal((2, 2, n))  # two pairs
+        ipath, opath = os.path.join(tmp, "gin.bin"), os.path.join(tmp, "gout.bin")
+        out.update({f"{name}_vx": vx, f"{name}_vy": vy, f"{name}_etov": etov, f"{name}_order": N,
+                    f"{name}_vin": vin})
+        for bc in (0, 1):
+            vin[:, 0].tofile(ipath)
+            run("grad", mpath, N, bc, ipath, opath)
+            out[f"{name}_grad_bc{bc}"] = np.fromfile(opath).reshape(2, 2, n)
+            vin.tofile(ipath)
+            run("div", mpath, N, bc, ipath, opath)
+            out[f"{name}_div_bc{bc}"] = np.fromfile(opath).reshape(2, n)
+    np.savez_compressed(os.path.join(HERE, "graddiv_ref.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["graddiv"]:
+        graddiv(tempfile.mkdtemp(prefix="dgc_golden_"))
+    else:
+        main()

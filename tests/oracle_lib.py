@@ -37,6 +37,12 @@ def lib(threaded=False):
                                  C.c_double, C.c_double, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
                                  C.c_double]
         L.oracle_rk4.restype = C.c_int
+        L.oracle_grad.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp,
+                                  _dp, _dp]
+        L.oracle_grad.restype = None
+        L.oracle_div.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_double,
+                                 _dp, _dp, _dp]
+        L.oracle_div.restype = None
         _LIBS[path] = L
     return _LIBS[path]
 
@@ -95,6 +101,27 @@ class Problem:
         self._lib.oracle_laplace_apply(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                    _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, _d(v))
         return v
+
+    def grad(self, u, g=None, beta=1.0):
+        """Weak gradient (grad_2d) with exterior value g[b] + beta * interior."""
+        u = np.ascontiguousarray(u, np.float64)
+        ox, oy = np.zeros_like(u), np.zeros_like(u)
+        gp = _d(np.ascontiguousarray(g, np.float64)) if g is not None else None
+        self._lib.oracle_grad(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx), _d(self.cy),
+                              _i(self.nb_cell), _i(self.nb_face), gp, beta, _d(u), _d(ox), _d(oy))
+        return ox, oy
+
+    def div(self, u0, u1, g0=None, g1=None, beta=1.0):
+        """Weak divergence (div_2d) with exterior values g_i[b] + beta * interior."""
+        u0 = np.ascontiguousarray(u0, np.float64)
+        u1 = np.ascontiguousarray(u1, np.float64)
+        out = np.zeros_like(u0)
+        g0 = np.ascontiguousarray(g0, np.float64) if g0 is not None else None
+        g1 = np.ascontiguousarray(g1, np.float64) if g1 is not None else None
+        self._lib.oracle_div(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx), _d(self.cy),
+                             _i(self.nb_cell), _i(self.nb_face), _d(g0) if g0 is not None else None,
+                             _d(g1) if g1 is not None else None, beta, _d(u0), _d(u1), _d(out))
+        return out
 
     def rk4(self, phys, u, t0, dt, nsteps=0, t_end=-1.0, ax=1.0, ay=1.0, bc=0):
         v = np.array(u, dtype=np.float64, copy=True)

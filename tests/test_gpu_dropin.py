@@ -48,3 +48,11 @@ def test_poisson_driver(tau_mode):
     out = run("poisson_2d", 3, "structured", tau_mode)
     m = re.search(r"max error = ([0-9.e+-]+)", out)
     assert m and float(m.group(1)) < 1e-3
+
+
+@pytest.mark.parametrize("order", [2, 3, 6])
+def test_graddiv_driver_exact_for_polynomials(order):
+    # dgc::grad_2d / dgc::div_2d with reference-style BC objects (Dirichlet data)
+    out = run("graddiv_2d", order)
+    m = re.search(r"grad max error = ([0-9.e+-]+), div max error = ([0-9.e+-]+)", out)
+    assert m and float(m.group(1)) < 1e-9 and float(m.group(2)) < 1e-9

@@ -160,3 +160,57 @@ def test_threaded_oracle_gives_the_same_bits():
         a, _ = P1.rk4(phys, u0, 0.0, 1e-3, nsteps=3, ax=1.0, ay=0.5, bc=1)
         b, _ = P2.rk4(phys, u0, 0.0, 1e-3, nsteps=3, ax=1.0, ay=0.5, bc=1)
         np.testing.assert_array_equal(a, b)
+
+
+def _graddiv_case(golden, name):
+    fix = golden("graddiv_ref.npz")
+    P, m = _problem(golden, fix, prefix=name + "_")
+    geo = m.geometry(P.N)
+    bx, by = geo["bx"], geo["by"]
+    # bc 1 of the fixture in affine form: exterior = g + beta * interior
+    g = {0: (None, None, 1.0), 1: (2 * np.sin(bx) * np.cos(by), 2 * np.cos(bx) * np.sin(by), -1.0)}
+    return fix, P, g
+
+
+@pytest.mark.parametrize("name", ["coarse", "grid"])
+def test_grad_and_div_match_reference(golden, name):
+    fix, P, g = _graddiv_case(golden, name)
+    for bc in (0, 1):
+        g0, g1, beta = g[bc]
+        for k in range(2):
+            u0, u1 = fix[f"{name}_vin"][k]
+            gx, gy = P.grad(u0, g0, beta)
+            want = fix[f"{name}_grad_bc{bc}"][k]
+            assert rel(gx, want[0]) < 1e-14 and rel(gy, want[1]) < 1e-14
+            assert rel(P.div(u0, u1, g0, g1, beta), fix[f"{name}_div_bc{bc}"][k]) < 1e-14
+
+
+def test_grad_planted_mistake_is_caught(golden):
+    """Dropping the boundary data (g = 0 instead of 2 sin(x) cos(y)) must break
+    the golden comparison."""
+    fix, P, g = _graddiv_case(golden, "coarse")
+    u0 = fix["coarse_vin"][0][0]
+    gx, _ = P.grad(u0, None, -1.0)
+    assert rel(gx, fix["coarse_grad_bc1"][0][0]) > 1e-3
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5])
+def test_grad_and_div_exact_for_polynomials(N):
+    """With exact boundary data the weak gradient of a polynomial of degree N
+    is its exact gradient (the central flux reproduces the continuous trace),
+    and likewise the divergence: a closed form the reference has no fixture for."""
+    m = dgc_amd.Mesh.structured(6, 5, -1, 1, -1, 1, jitter=0.15, seed=9)
+    P = Problem.from_mesh(m, N)
+    x, y = m.dof_coords(N, node_major=False)
+    geo = m.geometry(N)
+    bx, by = geo["bx"], geo["by"]
+    p = lambda x, y: x**N + 2 * x * y**(N - 1) - 3 * y**N + 0.5  # noqa: E731
+    px = lambda x, y: N * x**(N - 1) + 2 * y**(N - 1)  # noqa: E731
+    py = lambda x, y: 2 * (N - 1) * x * y**max(N - 2, 0) * (N > 1) - 3 * N * y**(N - 1)  # noqa: E731
+    gx, gy = P.grad(p(x, y), 2 * p(bx, by), -1.0)
+    np.testing.assert_allclose(gx, px(x, y), atol=1e-11)
+    np.testing.assert_allclose(gy, py(x, y), atol=1e-11)
+    q = lambda x, y: y**N - x * y**(N - 1)  # noqa: E731
+    qy = N * y**(N - 1) - (N - 1) * x * y**max(N - 2, 0)
+    d = P.div(p(x, y), q(x, y), 2 * p(bx, by), 2 * q(bx, by), -1.0)
+    np.testing.assert_allclose(d, px(x, y) + qy, atol=1e-11)
