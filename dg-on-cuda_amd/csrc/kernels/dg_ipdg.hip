@@ -161,7 +161,7 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass1(const IpdgParams<T> p) {
     if (valid) {
       const int nf = code[i] & 3, idx = code[i] >> 2;
       if (nf == kBoundaryCode) {
-        du = T(2) * a;
+        du = p.neumann ? T(0) : T(2) * a;
       } else {
         const int l = idx - c0;
         const T b = ((unsigned)l < (unsigned)kTile) ? us[Tr::fnode(nf, Nfp - 1 - d)][l] : far[i];
@@ -325,6 +325,8 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
           by = fary[i];
         }
         dqn = (qxs[own][lane] - bx) * vx + (qys[own][lane] - by) * vy;  // n.[q] face_J / J
+      } else if (p.neumann) {
+        dqn = T(2) * (qxs[own][lane] * vx + qys[own][lane] * vy);
       }
       v = (dqn + T(2) * (ftau[i] * fdu[i]) * fs) / T(2);
     }
@@ -569,6 +571,83 @@ hipError_t launch_mass_n(const IpdgParams<T>& p, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Inhomogeneous part of the Laplacian (src/laplace_2d.h:253-330,
+// apply_inhomogeneous_as_rhs), one thread per cell; cells without a boundary
+// face return at once. With g the Dirichlet data (gu, Dirichlet faces) and h
+// the Neumann data (hq, Neumann faces) at the boundary nodes of the cell:
+//   q     = -(1/J) LIFT (n g face_J / 2)               on the boundary faces
+//   inout += Dx qx + Dy qy - (1/J) LIFT ((h + 2 tau g) face_J / 2)
+// The reference lifts only the boundary faces in the second product too.
+template <int N>
+__global__ __launch_bounds__(256) void ipdg_inhomogeneous_kernel(const IpdgParams<double> p, const double* gu,
+                                                                  const double* hq, double* inout) {
+  using Tr = Tri<N>;
+  constexpr int Np = Tr::Np, Nfp = Tr::Nfp, NF = 3 * Nfp, Npad = ipdg_rows(Np);
+  const int K = p.K;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= K) return;
+  int code[3];
+  bool any = false;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    code[e] = p.nbr[(size_t)e * K + c];
+    any |= (code[e] & 3) == kBoundaryCode;
+  }
+  if (!any) return;
+  const double rx = p.geo[(size_t)kRx * K + c], ry = p.geo[(size_t)kRy * K + c];
+  const double sx = p.geo[(size_t)kSx * K + c], sy = p.geo[(size_t)kSy * K + c];
+  double f[NF];  // lift coefficients of the second product
+  double qx[Np], qy[Np];
+#pragma unroll
+  for (int j = 0; j < Np; ++j) qx[j] = qy[j] = 0.0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const bool bnd = (code[e] & 3) == kBoundaryCode;
+    const int off = code[e] >> 2;
+    double vx, vy;  // n face_J / J
+    face_vector(e, rx, ry, sx, sy, vx, vy);
+    const double fs = sqrt(vx * vx + vy * vy);
+    const double tau = p.tau[(size_t)e * K + c];
+#pragma unroll
+    for (int d = 0; d < Nfp; ++d) {
+      const int m = e * Nfp + d;
+      const double g = (bnd && !p.neumann && gu != nullptr) ? gu[off + d] : 0.0;
+      const double h = (bnd && p.neumann && hq != nullptr) ? hq[off + d] : 0.0;
+      f[m] = bnd ? (h + 2.0 * tau * g) * fs / 2.0 : 0.0;
+      const double lx = vx * g / 2.0, ly = vy * g / 2.0;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) {
+        const double l = p.LT[m * Npad + j];
+        qx[j] -= l * lx;
+        qy[j] -= l * ly;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < Np; ++j) {
+    double rqx = 0.0, sqx = 0.0, rqy = 0.0, sqy = 0.0;
+#pragma unroll
+    for (int k = 0; k < Np; ++k) {
+      const double a = p.DrT[k * Npad + j], b = p.DsT[k * Npad + j];
+      rqx = fma(a, qx[k], rqx);
+      sqx = fma(b, qx[k], sqx);
+      rqy = fma(a, qy[k], rqy);
+      sqy = fma(b, qy[k], sqy);
+    }
+    double v = rx * rqx + sx * sqx + ry * rqy + sy * sqy;
+#pragma unroll
+    for (int m = 0; m < NF; ++m) v -= p.LT[m * Npad + j] * f[m];
+    inout[(size_t)j * K + c] += v;
+  }
+}
+
+template <int N>
+hipError_t launch_inhomogeneous_n(const IpdgParams<double>& p, const double* gu, const double* hq, double* inout,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL((ipdg_inhomogeneous_kernel<N>), dim3((p.K + 255) / 256), dim3(256), 0, s, p, gu, hq, inout);
+  return hipGetLastError();
+}
+
 template <typename T, int N>
 hipError_t launch_update_n(const CgParams<T>& p, hipStream_t s) {
   const int grid = (p.K + kTile - 1) / kTile;
@@ -600,6 +679,13 @@ hipError_t ipdg_apply(int order, const IpdgParams<T>& p, int mode, hipStream_t s
 template <typename T>
 hipError_t ipdg_mass_apply(int order, const IpdgParams<T>& p, hipStream_t s) {
 #define CALL(n) launch_mass_n<T, n>(p, s)
+  DGC_ORDER_SWITCH(order, CALL)
+#undef CALL
+}
+
+hipError_t ipdg_inhomogeneous(int order, const IpdgParams<double>& p, const double* gu, const double* hq,
+                              double* inout, hipStream_t s) {
+#define CALL(n) launch_inhomogeneous_n<n>(p, gu, hq, inout, s)
   DGC_ORDER_SWITCH(order, CALL)
 #undef CALL
 }

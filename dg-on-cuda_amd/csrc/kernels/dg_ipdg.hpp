@@ -47,6 +47,7 @@ struct IpdgParams {
   const T* __restrict__ J;    // [K]
   const T* __restrict__ tau;  // [3][K] penalty per face
   const int* __restrict__ nbr;
+  int neumann;  // boundary faces: 0 Dirichlet (u+ = -u-, q+ = q-), 1 Neumann (u+ = u-, q+.n = -q-.n)
   const T* u;
   T* qx;
   T* qy;
@@ -160,6 +161,11 @@ template <typename T>
 hipError_t ipdg_apply(int order, const IpdgParams<T>& p, int mode, hipStream_t s);
 template <typename T>
 hipError_t ipdg_mass_apply(int order, const IpdgParams<T>& p, hipStream_t s);
+// inout += the inhomogeneous part of the Laplacian for boundary data gu
+// (Dirichlet exterior values at interior value 0) or hq (Neumann exterior
+// normal derivatives at 0), per boundary node; FP64 only.
+hipError_t ipdg_inhomogeneous(int order, const IpdgParams<double>& p, const double* gu, const double* hq,
+                              double* inout, hipStream_t s);
 template <typename T>
 hipError_t cg_update_launch(int order, const CgParams<T>& p, hipStream_t s);
 template <typename T>

@@ -177,6 +177,7 @@ dgc::IpdgParams<T> IpdgLevel<T>::params(const dgc_ipdg_s& s) const {
   pr.J = J_d;
   pr.tau = tau_d;
   pr.nbr = s.disc->nbr;
+  pr.neumann = s.neumann;
   pr.qx = qx;
   pr.qy = qy;
   pr.du = du;
@@ -511,6 +512,36 @@ int dgc_ipdg_apply_spd(dgc_ipdg_t s, const double* u, double* out) {
   return DGC_OK;
 }
 
+int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind) {
+  DGC_REQUIRE(s != nullptr, "dgc_ipdg_set_boundary: null solver");
+  DGC_REQUIRE(kind == DGC_BC_DIRICHLET || kind == DGC_BC_NEUMANN, "dgc_ipdg_set_boundary: unknown kind");
+  if (kind == s->neumann) return DGC_OK;
+  DGC_HIP_TRY(hipStreamSynchronize(s->disc->ctx->stream));
+  s->neumann = kind;
+  // everything derived from the operator is stale: block inverses, coarse
+  // space, captured solver graphs (they hold the old parameters)
+  release_coarse(s);
+  for (IpdgLevel<double>* L : {&s->f64}) {
+    (void)hipFree(L->P);
+    L->P = nullptr;
+    L->use_P = false;
+  }
+  (void)hipFree(s->f32.P);
+  s->f32.P = nullptr;
+  s->f32.use_P = false;
+  if (s->f32.graph) (void)hipGraphExecDestroy(s->f32.graph);
+  s->f32.graph = nullptr;
+  s->diag_blocks.clear();
+  return DGC_OK;
+}
+
+int dgc_ipdg_apply_inhomogeneous(dgc_ipdg_t s, const double* gu, const double* hq, double* inout) {
+  DGC_REQUIRE(s && inout, "dgc_ipdg_apply_inhomogeneous: null argument");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  DGC_LAUNCH(dgc::ipdg_inhomogeneous(s->disc->order, pr, gu, hq, inout, s->disc->ctx->stream));
+  return DGC_OK;
+}
+
 int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out) {
   DGC_REQUIRE(s && f && out && f != out, "dgc_ipdg_mass: bad argument");
   dgc::IpdgParams<double> pr = s->f64.params(*s);
@@ -645,6 +676,10 @@ extern "C" {
 int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell_y, int gx, int gy,
                           int* num_coarse) {
   DGC_REQUIRE(s && cell_x && cell_y, "dgc_ipdg_coarse_space: null argument");
+  if (s->neumann)
+    return dgc_set_error(DGC_ERR_UNSUPPORTED,
+                         "dgc_ipdg_coarse_space: the coarse space needs Dirichlet boundaries (with Neumann "
+                         "boundaries its coarse matrix is singular)");
   DGC_REQUIRE(gx >= 1 && gy >= 1, "dgc_ipdg_coarse_space: need gx, gy >= 1");
   dgc_disc_s* d = s->disc;
   const int K = d->K, Np = d->Np, N = d->order;

@@ -45,6 +45,7 @@ struct IpdgLevel {
 
 struct dgc_ipdg_s {
   dgc_disc_s* disc = nullptr;
+  int neumann = 0;  // boundary kind of every boundary face: 0 Dirichlet, 1 Neumann
   int tau_mode = 0;
   double tau_scale = 1.0;
   int grid = 0;

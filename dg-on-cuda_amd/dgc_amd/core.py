@@ -79,6 +79,8 @@ def _declare(L):
     d("dgc_rk4_profile_stages", [_vp, C.POINTER(_vp), C.c_size_t, C.c_double, C.c_double, C.c_int,
                                  C.POINTER(_vp), C.POINTER(C.c_float)])
     d("dgc_ipdg_create", [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.POINTER(_vp)])
+    d("dgc_ipdg_set_boundary", [_vp, C.c_int])
+    d("dgc_ipdg_apply_inhomogeneous", [_vp, _vp, _vp, _vp])
     d("dgc_ipdg_destroy", [_vp])
     d("dgc_ipdg_apply", [_vp, _vp])
     d("dgc_ipdg_apply_spd", [_vp, _vp, _vp])

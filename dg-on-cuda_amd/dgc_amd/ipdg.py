@@ -38,6 +38,23 @@ class IPDG:
     def apply_spd(self, u: DeviceArray, out: DeviceArray):
         check(lib().dgc_ipdg_apply_spd(self._h, _vp(u.ptr), _vp(out.ptr)))
 
+    DIRICHLET, NEUMANN = 0, 1
+
+    def set_boundary(self, kind: int):
+        """Boundary kind of every boundary face (the BC object's is_dirichlet()):
+        IPDG.DIRICHLET (default) or IPDG.NEUMANN. Drops the block-Jacobi
+        inverses and the coarse space, which depend on it."""
+        check(lib().dgc_ipdg_set_boundary(self._h, kind))
+        self.num_colors = None
+
+    def apply_inhomogeneous(self, inout: DeviceArray, gu: DeviceArray | None = None,
+                            hq: DeviceArray | None = None):
+        """inout += apply_inhomogeneous_as_rhs for boundary data per boundary node:
+        gu = exterior_val(x, y, 0) on Dirichlet faces, hq = exterior_grad_n(x, y, 0)
+        on Neumann faces (None: 0)."""
+        check(lib().dgc_ipdg_apply_inhomogeneous(self._h, _vp(gu.ptr) if gu is not None else None,
+                                                 _vp(hq.ptr) if hq is not None else None, _vp(inout.ptr)))
+
     def mass(self, f: DeviceArray, out: DeviceArray):
         check(lib().dgc_ipdg_mass(self._h, _vp(f.ptr), _vp(out.ptr)))
 

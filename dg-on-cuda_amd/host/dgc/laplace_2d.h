@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 
+#include "device_vector.h"
 #include "kernel_adapter.h"
 #include "simple_discretization_2d.h"
 
@@ -19,7 +20,7 @@ class laplace_2d : public simple_discretization_2d<T, M> {
  public:
   // tau_mode 0: tau = 1 as the reference; 1: (N+1)(N+2)/h per face
   laplace_2d(const M& mesh, int order, int tau_mode = 0, dgc_ctx_t ctx = default_context())
-      : simple_discretization_2d<T, M>(mesh, order), m_disc(create_on_device(*this, ctx)) {
+      : simple_discretization_2d<T, M>(mesh, order), m_ctx(ctx), m_disc(create_on_device(*this, ctx)) {
     const int Np = (order + 1) * (order + 2) / 2;
     m_Dr.resize(Np * Np);
     m_Ds.resize(Np * Np);
@@ -40,6 +41,40 @@ class laplace_2d : public simple_discretization_2d<T, M> {
     ka.execute_phase(phase_two{}, d_u);
   }
 
+  // The BC-object API of src/laplace_2d.h:52-62 on device vectors. The BC
+  // object (is_dirichlet, exterior_val, exterior_grad_n at (x, y)) is called
+  // on the host at every boundary node; one kind for all boundary faces.
+  template <typename BC2D>
+  void apply_homogeneous(T* d_u, const BC2D& bc) const {
+    set_kind(bc);
+    check(dgc_ipdg_apply(m_op->handle(), d_u), "apply failed");
+  }
+  // d_u += the inhomogeneous boundary part (src/laplace_2d.h:253-330)
+  template <typename BC2D>
+  void apply_inhomogeneous_as_rhs(T* d_u, const BC2D& bc) const {
+    const bool dirichlet = set_kind(bc);
+    const int nb = this->m_nbnodes;
+    if (nb == 0) return;
+    std::vector<T> h(nb);
+    for (int b = 0; b < nb; ++b)
+      h[b] = dirichlet ? bc.exterior_val(this->m_bx[b], this->m_by[b], T(0))
+                       : bc.exterior_grad_n(this->m_bx[b], this->m_by[b], T(0));
+    device_vector<T> g(h, m_ctx);
+    check(dgc_ipdg_apply_inhomogeneous(m_op->handle(), dirichlet ? g.data() : nullptr,
+                                       dirichlet ? nullptr : g.data(), d_u),
+          "inhomogeneous apply failed");
+  }
+  // in place: u <- Lap_h u with the BC's data (src/laplace_2d.h:121-128):
+  // the homogeneous part minus the inhomogeneous one
+  template <typename BC2D>
+  void operator()(T* d_u, const BC2D& bc) const {
+    apply_homogeneous(d_u, bc);
+    device_vector<T> z(this->total_num_nodes(), m_ctx);
+    check(dgc_memset_zero(m_ctx, z.data(), z.size() * sizeof(T)), "memset failed");
+    apply_inhomogeneous_as_rhs(z.data(), bc);
+    check(dgc_axpy(m_ctx, -1.0, z.data(), z.size(), d_u, d_u), "axpy failed");
+  }
+
   // -Lap_h u = f: b = M f, then block-Jacobi PCG to the relative residual rtol.
   // Returns the iteration count.
   int solve(const T* d_f, T* d_u, double rtol = 1e-10, int maxit = 100000) const {
@@ -55,6 +90,21 @@ class laplace_2d : public simple_discretization_2d<T, M> {
   dgc_ipdg_t handle() const { return m_op->handle(); }
 
  private:
+  // boundary kind of the BC object (one kind on every boundary node); true: Dirichlet
+  template <typename BC2D>
+  bool set_kind(const BC2D& bc) const {
+    bool dirichlet = true;
+    for (int b = 0; b < this->m_nbnodes; ++b) {
+      const bool d = bc.is_dirichlet(this->m_bx[b], this->m_by[b]);
+      if (b == 0) dirichlet = d;
+      if (d != dirichlet) throw std::invalid_argument("laplace_2d: mixed Dirichlet/Neumann boundaries");
+    }
+    check(dgc_ipdg_set_boundary(m_op->handle(), dirichlet ? DGC_BC_DIRICHLET : DGC_BC_NEUMANN),
+          "boundary kind failed");
+    return dirichlet;
+  }
+
+  dgc_ctx_t m_ctx;
   device_discretization m_disc;
   std::unique_ptr<device_laplace_op> m_op;
   std::vector<T> m_Dr, m_Ds, m_L, m_M;

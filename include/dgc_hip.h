@@ -233,14 +233,16 @@ int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, dou
  * [Np*Np] and lift [Np*3Nfp] are the strong-form matrices, mass [Np*Np] is
  * (V V^T)^{-1}. tau_mode 0: the reference's tau = 1 (src/laplace_2d.h:227-245);
  * tau_mode 1: tau = (N+1)(N+2)/h per face, h = min over both sides of
- * 2J/face_J. tau_scale multiplies either. Every boundary face is homogeneous
- * Dirichlet (examples/poisson_2d/poisson_2d.h:73-79). */
+ * 2J/face_J. tau_scale multiplies either. Boundary faces are Dirichlet
+ * (examples/poisson_2d/poisson_2d.h:73-79) unless dgc_ipdg_set_boundary says
+ * Neumann. */
 typedef struct dgc_ipdg_s* dgc_ipdg_t;
 int dgc_ipdg_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
                     const double* mass, int tau_mode, double tau_scale, dgc_ipdg_t* out);
 int dgc_ipdg_destroy(dgc_ipdg_t s);
-/* In place u <- Lap_h u: laplace_2d::operator() with homogeneous Dirichlet
- * data (src/laplace_2d.h:121-128); replaces op_on_device
+/* In place u <- Lap_h u: laplace_2d::apply_homogeneous (src/laplace_2d.h:130-251;
+ * with Dirichlet faces and zero data this is laplace_2d::operator(),
+ * src/laplace_2d.h:121-128); replaces op_on_device
  * (examples/poisson_2d/k_poisson_2d.cu:99-108). u: K*Np node-major. */
 int dgc_ipdg_apply(dgc_ipdg_t s, double* u);
 /* One phase of the in-place apply, as launched by kernel_adapter::execute_phase
@@ -252,6 +254,23 @@ int dgc_ipdg_phase(dgc_ipdg_t s, int phase, double* u);
 int dgc_ipdg_apply_spd(dgc_ipdg_t s, const double* u, double* out);
 /* out = M f (mass-weighted right-hand side of -Lap u = f). */
 int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out);
+/* Boundary kind of every boundary face, the BC object's is_dirichlet() of
+ * src/laplace_2d.h:175,218 (default DGC_BC_DIRICHLET): Dirichlet faces take
+ * u+ = -u-, q+ = q- in the homogeneous apply; Neumann faces u+ = u-,
+ * q+.n = -q-.n (no penalty). Switching drops the block-Jacobi inverses and
+ * the coarse space (rebuilt on demand; the coarse space is Dirichlet-only).
+ * With Neumann faces everywhere the operator is singular (constants): CG and
+ * block-Jacobi PCG need a right-hand side orthogonal to the constants. */
+enum { DGC_BC_DIRICHLET = 0, DGC_BC_NEUMANN = 1 };
+int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind);
+/* inout += the inhomogeneous boundary part of the Laplacian
+ * (laplace_2d::apply_inhomogeneous_as_rhs, src/laplace_2d.h:253-330) for
+ * boundary data per boundary node (device arrays in dgc_disc2d_arrays' bx
+ * order; NULL = 0): gu = the BC object's exterior_val(x, y, 0) on Dirichlet
+ * faces, hq = exterior_grad_n(x, y, 0) on Neumann faces. The full operator
+ * laplace_2d::operator()(u, bc) is dgc_ipdg_apply(u) followed by subtracting
+ * this part (src/laplace_2d.h:121-128). */
+int dgc_ipdg_apply_inhomogeneous(dgc_ipdg_t s, const double* gu, const double* hq, double* inout);
 /* Builds the block-Jacobi preconditioner: the exact diagonal blocks of
  * -M Lap_h, probed with a distance-2 colouring of the cells, inverted per cell. */
 int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);

@@ -19,6 +19,7 @@
 // threaded build gives the same bits); liboracle.so is built without
 // -fopenmp (serial, as the reference), liboracle_omp.so with it, for the
 // all-core CPU baseline.
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -295,9 +296,22 @@ static double face_tau(int tau_mode, int N, const double* cx, const double* cy, 
 // src/laplace_2d.h:121-251 with the BC of examples/poisson_2d/poisson_2d.h:73-79
 // (every boundary is Dirichlet; the inhomogeneous part is zero for g = 0).
 // Strong-form Dr, Ds. u: K*Np cell-major, overwritten by Delta_h u.
+void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, const double* L,
+                             const double* cx, const double* cy, const int* nb_cell,
+                             const int* nb_face, int tau_mode, int neumann, double* u);
+
 void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, const double* L,
                           const double* cx, const double* cy, const int* nb_cell,
                           const int* nb_face, int tau_mode, double* u) {
+  oracle_laplace_apply_bc(K, N, Dr, Ds, L, cx, cy, nb_cell, nb_face, tau_mode, 0, u);
+}
+
+// laplace_2d::apply_homogeneous (src/laplace_2d.h:130-251) with one boundary
+// kind for every boundary node: neumann 0 is bc.is_dirichlet() == true
+// (du = 2u, dqN = 0), neumann 1 is false (du = 0, dqN = 2 q.n).
+void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, const double* L,
+                             const double* cx, const double* cy, const int* nb_cell,
+                             const int* nb_face, int tau_mode, int neumann, double* u) {
   const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
   std::vector<double> qx(size_t(K) * Np), qy(size_t(K) * Np), du(size_t(K) * 3 * Nfp);
   std::vector<double> Dx(Np * Np), Dy(Np * Np), tlx(3 * Nfp), tly(3 * Nfp);
@@ -320,7 +334,7 @@ void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, cons
       for (int i = 0; i < Nfp; ++i) {
         double d;
         if (isB)
-          d = 2.0 * u[o + face_node(N, e, i)];
+          d = neumann ? 0.0 : 2.0 * u[o + face_node(N, e, i)];
         else
           d = u[o + face_node(N, e, i)] - u[size_t(nbc) * Np + face_node(N, nbf, Nfp - i - 1)];
         du[size_t(c) * 3 * Nfp + e * Nfp + i] = d;
@@ -353,7 +367,7 @@ void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, cons
         const int a = face_node(N, e, i);
         double dqN;
         if (isB) {
-          dqN = 0.0;
+          dqN = neumann ? 2.0 * (qx[o + a] * nx + qy[o + a] * ny) : 0.0;
         } else {
           const size_t b = size_t(nbc) * Np + face_node(N, nbf, Nfp - i - 1);
           dqN = (qx[o + a] - qx[b]) * nx + (qy[o + a] - qy[b]) * ny;
@@ -363,6 +377,61 @@ void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, cons
       }
     }
     gemv(L, Np, 3 * Nfp, -1.0 / cell_J(cell), tlx.data(), 1.0, u + o);
+  }
+}
+
+static std::vector<int> boundary_offsets(int K, int Nfp, const int* nb_cell);
+
+// laplace_2d::apply_inhomogeneous_as_rhs (src/laplace_2d.h:253-330): inout +=
+// the boundary part for data gu (exterior_val(x, y, 0) on Dirichlet faces) or
+// hq (exterior_grad_n(x, y, 0) on Neumann faces) per boundary node (NULL = 0),
+// one boundary kind for every face as oracle_laplace_apply_bc.
+void oracle_laplace_inhomogeneous(int K, int N, const double* Dr, const double* Ds, const double* L,
+                                  const double* cx, const double* cy, const int* nb_cell, const int* nb_face,
+                                  int tau_mode, int neumann, const double* gu, const double* hq, double* inout) {
+  const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  const std::vector<int> boff = boundary_offsets(K, Nfp, nb_cell);
+  std::vector<double> qx(Np), qy(Np), du(3 * Nfp), Dx(Np * Np), Dy(Np * Np), tlx(3 * Nfp), tly(3 * Nfp);
+  for (int c = 0; c < K; ++c) {
+    if (nb_cell[3 * c] != c && nb_cell[3 * c + 1] != c && nb_cell[3 * c + 2] != c) continue;
+    const Cell cell = get_cell(cx, cy, c);
+    std::fill(tlx.begin(), tlx.end(), 0.0);
+    std::fill(tly.begin(), tly.end(), 0.0);
+    std::fill(du.begin(), du.end(), 0.0);
+    for (int e = 0; e < 3; ++e) {
+      if (nb_cell[3 * c + e] != c) continue;
+      double nx, ny;
+      outward_normal(cell, e, nx, ny);
+      const double fj = face_J(cell, e);
+      for (int i = 0; i < Nfp; ++i) {
+        const double g = (!neumann && gu) ? gu[boff[3 * c + e] + i] : 0.0;
+        du[e * Nfp + i] = g;
+        tlx[e * Nfp + i] = nx * g * fj / 2.0;
+        tly[e * Nfp + i] = ny * g * fj / 2.0;
+      }
+    }
+    const double J = cell_J(cell);
+    gemv(L, Np, 3 * Nfp, -1.0 / J, tlx.data(), 0.0, qx.data());
+    gemv(L, Np, 3 * Nfp, -1.0 / J, tly.data(), 0.0, qy.data());
+    double ij[4];
+    inv_jacobian(cell, ij);
+    combine(Dr, Ds, Np * Np, ij[0], ij[2], Dx.data());
+    combine(Dr, Ds, Np * Np, ij[1], ij[3], Dy.data());
+    const size_t o = size_t(c) * Np;
+    gemv(Dx.data(), Np, Np, 1.0, qx.data(), 1.0, inout + o);
+    gemv(Dy.data(), Np, Np, 1.0, qy.data(), 1.0, inout + o);
+    std::fill(tlx.begin(), tlx.end(), 0.0);
+    for (int e = 0; e < 3; ++e) {
+      if (nb_cell[3 * c + e] != c) continue;
+      const double fj = face_J(cell, e);
+      const double tau = face_tau(tau_mode, N, cx, cy, nb_cell, nb_face, c, e);
+      for (int i = 0; i < Nfp; ++i) {
+        const double h = (neumann && hq) ? hq[boff[3 * c + e] + i] : 0.0;
+        const double d = du[e * Nfp + i];
+        tlx[e * Nfp + i] = (h + 2.0 * (tau_mode == 0 ? d : tau * d)) * fj / 2.0;
+      }
+    }
+    gemv(L, Np, 3 * Nfp, -1.0 / J, tlx.data(), 1.0, inout + o);
   }
 }
 

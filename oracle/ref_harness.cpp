@@ -352,6 +352,48 @@ int main(int argc, char** argv) {
     return 0;
   }
 
+  if (cmd == "laplace_bc" && argc == 7) {
+    // laplace_2d (src/laplace_2d.h:44-81) with a BC object of one kind on every
+    // boundary node. kind 0 (Dirichlet): exterior_val(x, y, a) = 2 sin(x) cos(y) - a;
+    // kind 1 (Neumann): exterior_grad_n(x, y, g) = 2 (cos(x) sin(y) + x y) - g.
+    // args: mesh N kind infile out. out: apply_homogeneous of every input vector,
+    // then apply_inhomogeneous_as_rhs of a zero vector, then operator() of every vector.
+    Mesh m = load_mesh(argv[2]);
+    const int N = std::atoi(argv[3]);
+    const int kind = std::atoi(argv[4]);
+    struct BC {
+      int kind;
+      bool is_dirichlet(double, double) const { return kind == 0; }
+      double exterior_val(double x, double y, double a) const { return kind == 0 ? 2.0 * std::sin(x) * std::cos(y) - a : a; }
+      double exterior_grad_n(double x, double y, double g) const {
+        return kind == 0 ? g : 2.0 * (std::cos(x) * std::sin(y) + x * y) - g;
+      }
+    };
+    const BC bc{kind};
+    dgc::laplace_2d<double, Mesh> op(m, N);
+    const std::size_t n = op.total_num_nodes();
+    std::ifstream in(argv[5], std::ios::binary | std::ios::ate);
+    const std::size_t bytes = in.tellg();
+    in.seekg(0);
+    if (bytes % (8 * n) != 0) die("laplace_bc: input size is not a multiple of K*Np");
+    const std::size_t nvec = bytes / (8 * n);
+    std::vector<std::vector<double>> vin(nvec, std::vector<double>(n));
+    for (auto& v : vin) in.read((char*)v.data(), 8 * n);
+    std::ofstream f(argv[6], std::ios::binary);
+    for (auto v : vin) {
+      op.apply_homogeneous(v.begin(), bc);
+      write(f, v);
+    }
+    std::vector<double> z(n, 0.0);
+    op.apply_inhomogeneous_as_rhs(z.begin(), bc);
+    write(f, z);
+    for (auto v : vin) {
+      op(v.begin(), bc);
+      write(f, v);
+    }
+    return 0;
+  }
+
   if (cmd == "laplace_time" && argc == 5) {
     // Timing only: args mesh N napply; prints "napply seconds".
     Mesh m = load_mesh(argv[2]);

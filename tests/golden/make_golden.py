@@ -136,6 +136,7 @@ def main():
                         vin=vin, vout=vout, coarse_vx=cvx, coarse_vy=cvy, coarse_etov=cetov,
                         coarse_order=2, coarse_A=A)
     graddiv(tmp)
+    laplace_bc(tmp)
     print("golden vectors written to", HERE)
 
 
@@ -167,8 +168,38 @@ def graddiv(tmp):
     np.savez_compressed(os.path.join(HERE, "graddiv_ref.npz"), **out)
 
 
+def laplace_bc(tmp):
+    """6. laplace_2d with a BC object of one kind on every boundary node (kind 0
+    Dirichlet, exterior_val = 2 sin(x) cos(y) - a; kind 1 Neumann,
+    exterior_grad_n = 2 (cos(x) sin(y) + x y) - g): apply_homogeneous of two seeded
+    vectors, apply_inhomogeneous_as_rhs of zero, operator() of the two vectors; on
+    a jittered 8 x 6 mesh of [-1, 1]^2 at N = 3 and the K = 16 gmsh mesh at N = 2."""
+    out = {}
+    meshes = {"grid": (dgc.Mesh.structured(8, 6, -1, 1, -1, 1, jitter=0.15, seed=12), 3),
+              "coarse": (dgc.Mesh.read_gmsh(os.path.join(HERE, "square_domain_coarse.msh")), 2)}
+    for name, (m, N) in meshes.items():
+        vx, vy, etov = mesh_arrays(m)
+        mpath = os.path.join(tmp, f"lb_{name}.bin")
+        write_mesh(mpath, vx, vy, etov)
+        n = m.K * (N + 1) * (N + 2) // 2
+        vin = np.random.default_rng(17).standard_normal((2, n))
+        ipath, opath = os.path.join(tmp, "lbin.bin"), os.path.join(tmp, "lbout.bin")
+        vin.tofile(ipath)
+        out.update({f"{name}_vx": vx, f"{name}_vy": vy, f"{name}_etov": etov, f"{name}_order": N,
+                    f"{name}_vin": vin})
+        for kind in (0, 1):
+            run("laplace_bc", mpath, N, kind, ipath, opath)
+            r = np.fromfile(opath).reshape(5, n)
+            out[f"{name}_hom_k{kind}"] = r[:2]
+            out[f"{name}_inh_k{kind}"] = r[2]
+            out[f"{name}_full_k{kind}"] = r[3:]
+    np.savez_compressed(os.path.join(HERE, "laplace_bc_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["graddiv"]:
         graddiv(tempfile.mkdtemp(prefix="dgc_golden_"))
+    elif sys.argv[1:] == ["laplace_bc"]:
+        laplace_bc(tempfile.mkdtemp(prefix="dgc_golden_"))
     else:
         main()

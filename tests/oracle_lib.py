@@ -37,6 +37,12 @@ def lib(threaded=False):
                                  C.c_double, C.c_double, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
                                  C.c_double]
         L.oracle_rk4.restype = C.c_int
+        L.oracle_laplace_apply_bc.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
+                                              C.c_int, _dp]
+        L.oracle_laplace_apply_bc.restype = None
+        L.oracle_laplace_inhomogeneous.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
+                                                   C.c_int, _dp, _dp, _dp]
+        L.oracle_laplace_inhomogeneous.restype = None
         L.oracle_grad.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp,
                                   _dp, _dp]
         L.oracle_grad.restype = None
@@ -100,6 +106,24 @@ class Problem:
         v = np.array(u, dtype=np.float64, copy=True)
         self._lib.oracle_laplace_apply(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                    _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, _d(v))
+        return v
+
+    def laplace_bc(self, u, neumann, tau_mode=0):
+        """apply_homogeneous with one boundary kind (0 Dirichlet, 1 Neumann)."""
+        v = np.array(u, dtype=np.float64, copy=True)
+        self._lib.oracle_laplace_apply_bc(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                          _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, neumann, _d(v))
+        return v
+
+    def laplace_inhomogeneous(self, inout, neumann, gu=None, hq=None, tau_mode=0):
+        """inout + apply_inhomogeneous_as_rhs for boundary data gu (Dirichlet) or hq (Neumann)."""
+        v = np.array(inout, dtype=np.float64, copy=True)
+        gu = np.ascontiguousarray(gu, np.float64) if gu is not None else None
+        hq = np.ascontiguousarray(hq, np.float64) if hq is not None else None
+        self._lib.oracle_laplace_inhomogeneous(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                               _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, neumann,
+                                               _d(gu) if gu is not None else None,
+                                               _d(hq) if hq is not None else None, _d(v))
         return v
 
     def grad(self, u, g=None, beta=1.0):

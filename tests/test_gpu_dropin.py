@@ -56,3 +56,28 @@ def test_graddiv_driver_exact_for_polynomials(order):
     out = run("graddiv_2d", order)
     m = re.search(r"grad max error = ([0-9.e+-]+), div max error = ([0-9.e+-]+)", out)
     assert m and float(m.group(1)) < 1e-9 and float(m.group(2)) < 1e-9
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_laplace_driver_with_bc_objects(order):
+    # dgc::laplace_2d::operator()(u, bc) with Dirichlet and Neumann BC objects
+    # against the oracle (apply_homogeneous minus the inhomogeneous part)
+    import numpy as np
+
+    import dgc_amd
+    from oracle_lib import Problem
+
+    out = run("laplace_bc_2d", order)
+    m = re.search(r"dirichlet norm = ([0-9.e+-]+), neumann norm = ([0-9.e+-]+)", out)
+    mesh = dgc_amd.Mesh.structured(10, 8, -1, 1, -1, 1, jitter=0.15, seed=7)
+    P = Problem.from_mesh(mesh, order, form="strong")
+    x, y = mesh.dof_coords(order, node_major=False)
+    u = np.sin(x) * np.cos(y)
+    geo = mesh.geometry(order)
+    bx, by = geo["bx"], geo["by"]
+    gu = 2 * np.sin(bx) * np.cos(by)
+    hq = 2 * (np.cos(bx) * np.sin(by) + bx * by)
+    zero = np.zeros_like(u)
+    for kind, got in ((0, float(m.group(1))), (1, float(m.group(2)))):
+        want = np.linalg.norm(P.laplace_bc(u, kind) - P.laplace_inhomogeneous(zero, kind, gu, hq))
+        assert abs(got - want) <= 1e-12 * want

@@ -214,3 +214,42 @@ def test_grad_and_div_exact_for_polynomials(N):
     qy = N * y**(N - 1) - (N - 1) * x * y**max(N - 2, 0)
     d = P.div(p(x, y), q(x, y), 2 * p(bx, by), 2 * q(bx, by), -1.0)
     np.testing.assert_allclose(d, px(x, y) + qy, atol=1e-11)
+
+
+def _laplace_bc_case(golden, name):
+    fix = golden("laplace_bc_ref.npz")
+    P, m = _problem(golden, fix, form="strong", prefix=name + "_")
+    geo = m.geometry(P.N)
+    bx, by = geo["bx"], geo["by"]
+    gu = 2 * np.sin(bx) * np.cos(by)               # Dirichlet: exterior_val(x, y, 0)
+    hq = 2 * (np.cos(bx) * np.sin(by) + bx * by)   # Neumann: exterior_grad_n(x, y, 0)
+    return fix, P, gu, hq
+
+
+@pytest.mark.parametrize("name", ["grid", "coarse"])
+def test_laplace_boundary_kinds_match_reference(golden, name):
+    fix, P, gu, hq = _laplace_bc_case(golden, name)
+    zero = np.zeros(P.K * P.Np)
+    for kind in (0, 1):
+        inh = P.laplace_inhomogeneous(zero, kind, gu, hq)
+        assert rel(inh, fix[f"{name}_inh_k{kind}"]) < 1e-14
+        for k in range(2):
+            hom = P.laplace_bc(fix[f"{name}_vin"][k], kind)
+            assert rel(hom, fix[f"{name}_hom_k{kind}"][k]) < 1e-14
+            # operator() = apply_homogeneous - inhomogeneous part (src/laplace_2d.h:121-128)
+            assert rel(hom - inh, fix[f"{name}_full_k{kind}"][k]) < 1e-13
+
+
+def test_laplace_boundary_kind_planted_mistake_is_caught(golden):
+    fix, P, gu, hq = _laplace_bc_case(golden, "grid")
+    wrong = P.laplace_bc(fix["grid_vin"][0], 0)  # Dirichlet where the fixture is Neumann
+    assert rel(wrong, fix["grid_hom_k1"][0]) > 1e-3
+
+
+def test_neumann_laplacian_annihilates_constants():
+    """With Neumann faces everywhere the constants span the null space."""
+    m = dgc_amd.Mesh.structured(7, 6, -1, 1, -1, 1, jitter=0.15, seed=2)
+    for N in (1, 3, 5):
+        P = Problem.from_mesh(m, N, form="strong")
+        assert np.abs(P.laplace_bc(np.ones(P.K * P.Np), 1)).max() < 1e-10
+        assert np.abs(P.laplace_bc(np.ones(P.K * P.Np), 0)).max() > 1.0
