@@ -161,7 +161,7 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass1(const IpdgParams<T> p) {
     if (valid) {
       const int nf = code[i] & 3, idx = code[i] >> 2;
       if (nf == kBoundaryCode) {
-        du = p.neumann ? T(0) : T(2) * a;
+        du = p.neumann ? T(0) : T(2) * a - (p.gbd != nullptr ? T(p.gbd[idx + d]) : T(0));
       } else {
         const int l = idx - c0;
         const T b = ((unsigned)l < (unsigned)kTile) ? us[Tr::fnode(nf, Nfp - 1 - d)][l] : far[i];
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
         }
         dqn = (qxs[own][lane] - bx) * vx + (qys[own][lane] - by) * vy;  // n.[q] face_J / J
       } else if (p.neumann) {
-        dqn = T(2) * (qxs[own][lane] * vx + qys[own][lane] * vy);
+        dqn = T(2) * (qxs[own][lane] * vx + qys[own][lane] * vy) - (p.hbd != nullptr ? T(p.hbd[idx + d]) * fs : T(0));
       }
       v = (dqn + T(2) * (ftau[i] * fdu[i]) * fs) / T(2);
     }

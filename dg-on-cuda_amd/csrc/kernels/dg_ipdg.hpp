@@ -48,6 +48,11 @@ struct IpdgParams {
   const T* __restrict__ tau;  // [3][K] penalty per face
   const int* __restrict__ nbr;
   int neumann;  // boundary faces: 0 Dirichlet (u+ = -u-, q+ = q-), 1 Neumann (u+ = u-, q+.n = -q-.n)
+  // optional boundary data per boundary node (null: homogeneous): the exterior
+  // value at interior 0 on Dirichlet faces (u+ = gbd - u-) and the exterior
+  // normal derivative at interior 0 on Neumann faces (q+.n = hbd - q-.n)
+  const double* gbd;
+  const double* hbd;
   const T* u;
   T* qx;
   T* qy;

@@ -535,6 +535,17 @@ int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind) {
   return DGC_OK;
 }
 
+int dgc_ipdg_apply_data(dgc_ipdg_t s, const double* gd, const double* hd, double* u) {
+  DGC_REQUIRE(s && u, "dgc_ipdg_apply_data: null argument");
+  dgc::IpdgParams<double> pr = s->f64.params(*s);
+  pr.u = u;
+  pr.out = u;
+  pr.gbd = gd;
+  pr.hbd = hd;
+  DGC_LAUNCH(dgc::ipdg_apply<double>(s->disc->order, pr, dgc::kLaplace, s->disc->ctx->stream));
+  return DGC_OK;
+}
+
 int dgc_ipdg_apply_inhomogeneous(dgc_ipdg_t s, const double* gu, const double* hq, double* inout) {
   DGC_REQUIRE(s && inout, "dgc_ipdg_apply_inhomogeneous: null argument");
   dgc::IpdgParams<double> pr = s->f64.params(*s);

@@ -63,6 +63,10 @@ def _declare(L):
     d("dgc_graddiv_destroy", [_vp])
     d("dgc_grad2d_apply", [_vp, _vp, _vp, C.c_double, _vp, _vp])
     d("dgc_div2d_apply", [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
+    d("dgc_stokes_create", [_vp, C.POINTER(_vp)])
+    d("dgc_stokes_destroy", [_vp])
+    d("dgc_stokes_advance", [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp,
+                             _ip])
     d("dgc_maxwell_tm_create", [_vp, _dp, _dp, _dp, C.POINTER(_vp)])
     d("dgc_advection_create", [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)])
     d("dgc_op_destroy", [_vp])
@@ -81,6 +85,7 @@ def _declare(L):
     d("dgc_ipdg_create", [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.POINTER(_vp)])
     d("dgc_ipdg_set_boundary", [_vp, C.c_int])
     d("dgc_ipdg_apply_inhomogeneous", [_vp, _vp, _vp, _vp])
+    d("dgc_ipdg_apply_data", [_vp, _vp, _vp, _vp])
     d("dgc_ipdg_destroy", [_vp])
     d("dgc_ipdg_apply", [_vp, _vp])
     d("dgc_ipdg_apply_spd", [_vp, _vp, _vp])

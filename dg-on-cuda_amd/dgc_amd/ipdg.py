@@ -55,6 +55,12 @@ class IPDG:
         check(lib().dgc_ipdg_apply_inhomogeneous(self._h, _vp(gu.ptr) if gu is not None else None,
                                                  _vp(hq.ptr) if hq is not None else None, _vp(inout.ptr)))
 
+    def apply_data(self, u: DeviceArray, gd: DeviceArray | None = None, hd: DeviceArray | None = None):
+        """In place u <- Lap_h u with boundary data (the whole BR1 operator):
+        Dirichlet faces u+ = gd - u-, Neumann faces q+.n = hd - q-.n."""
+        check(lib().dgc_ipdg_apply_data(self._h, _vp(gd.ptr) if gd is not None else None,
+                                        _vp(hd.ptr) if hd is not None else None, _vp(u.ptr)))
+
     def mass(self, f: DeviceArray, out: DeviceArray):
         check(lib().dgc_ipdg_mass(self._h, _vp(f.ptr), _vp(out.ptr)))
 

@@ -38,6 +38,7 @@ enum dgc_status {
 typedef struct dgc_ctx_s* dgc_ctx_t;
 typedef struct dgc_disc_s* dgc_disc_t;
 typedef struct dgc_graddiv_s* dgc_graddiv_t;
+typedef struct dgc_stokes_s* dgc_stokes_t;
 typedef struct dgc_op_s* dgc_op_t;
 typedef struct dgc_mesh_s* dgc_mesh_t;
 
@@ -151,6 +152,24 @@ int dgc_grad2d_apply(dgc_graddiv_t op, const double* u, const double* g, double 
  * boundary data of the two components */
 int dgc_div2d_apply(dgc_graddiv_t op, const double* u0, const double* u1, const double* g0,
                     const double* g1, double beta, double* out);
+
+/* ---- Stokes dual-splitting step (examples/stokes_2d) ---------------------
+ * stokes_2d::advance_timestep (examples/stokes_2d/stokes_2d.h:358-386) on the
+ * GPU, the path the reference leaves as a TODO (k_stokes_2d.cu:96-106): from
+ * (u^{n-1}, t_prev) and (u^n, t) to u^{n+1} at t + dt for the example's
+ * analytic problem (a = 2.883356; its boundary data, stokes_2d.h:100-183),
+ * with tau = 1 as the reference. Pressure: block-Jacobi PCG on -M Lap_h with
+ * Neumann faces, right-hand side projected onto the complement of the
+ * constants (the reference factorises the singular matrix directly);
+ * viscous step: block-Jacobi PCG on -M Lap_h + (3 / 2dt) M with Dirichlet
+ * faces, one solve per component. rtol: relative residual of every solve.
+ * Velocities: device arrays, node-major [Np][K]. iters (may be NULL): the
+ * pressure, x and y iteration counts. Synchronous. */
+int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out);
+int dgc_stokes_destroy(dgc_stokes_t s);
+int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_prev, double t_prev,
+                       const double* ux, const double* uy, double t, double dt, double rtol,
+                       double* ux_out, double* uy_out, int* iters);
 
 /* ---- semi-discrete operators (device objects) ----------------------------
  * Replace create_device_object / destroy_device_object of
@@ -271,6 +290,16 @@ int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind);
  * laplace_2d::operator()(u, bc) is dgc_ipdg_apply(u) followed by subtracting
  * this part (src/laplace_2d.h:121-128). */
 int dgc_ipdg_apply_inhomogeneous(dgc_ipdg_t s, const double* gu, const double* hq, double* inout);
+/* In place u <- Lap_h u with boundary data, the whole BR1 operator: Dirichlet
+ * faces take u+ = gd - u- (gd = exterior_val(x, y, 0)), Neumann faces
+ * q+.n = hd - q-.n (hd = exterior_grad_n(x, y, 0)); device arrays per boundary
+ * node, NULL = 0 (then it is dgc_ipdg_apply). Its affine part (the call on
+ * u = 0) also reaches the interior faces of boundary cells, which
+ * dgc_ipdg_apply_inhomogeneous leaves out as the reference does: with
+ * inhomogeneous Dirichlet data only this form is consistent (the reference's
+ * operator()(u, bc) error grows as the mesh is refined). No counterpart in the
+ * reference; the Stokes step uses it. */
+int dgc_ipdg_apply_data(dgc_ipdg_t s, const double* gd, const double* hd, double* u);
 /* Builds the block-Jacobi preconditioner: the exact diagonal blocks of
  * -M Lap_h, probed with a distance-2 colouring of the cells, inverted per cell. */
 int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);

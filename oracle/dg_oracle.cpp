@@ -296,6 +296,10 @@ static double face_tau(int tau_mode, int N, const double* cx, const double* cy, 
 // src/laplace_2d.h:121-251 with the BC of examples/poisson_2d/poisson_2d.h:73-79
 // (every boundary is Dirichlet; the inhomogeneous part is zero for g = 0).
 // Strong-form Dr, Ds. u: K*Np cell-major, overwritten by Delta_h u.
+void oracle_laplace_apply_data(int K, int N, const double* Dr, const double* Ds, const double* L,
+                               const double* cx, const double* cy, const int* nb_cell, const int* nb_face,
+                               int tau_mode, int neumann, const double* gd, const double* hd, double* u);
+static std::vector<int> boundary_offsets(int K, int Nfp, const int* nb_cell);
 void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, const double* L,
                              const double* cx, const double* cy, const int* nb_cell,
                              const int* nb_face, int tau_mode, int neumann, double* u);
@@ -312,7 +316,20 @@ void oracle_laplace_apply(int K, int N, const double* Dr, const double* Ds, cons
 void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, const double* L,
                              const double* cx, const double* cy, const int* nb_cell,
                              const int* nb_face, int tau_mode, int neumann, double* u) {
+  oracle_laplace_apply_data(K, N, Dr, Ds, L, cx, cy, nb_cell, nb_face, tau_mode, neumann, nullptr, nullptr, u);
+}
+
+// The same two passes with boundary data (gd: the exterior value at interior
+// value 0 on Dirichlet faces, u+ = gd - u-; hd: the exterior normal
+// derivative at interior 0 on Neumann faces, q+.n = hd - q-.n; per boundary
+// node, NULL = 0): the whole BR1 operator with inhomogeneous data, whose
+// affine part also reaches the interior faces of boundary cells (which
+// laplace_2d::apply_inhomogeneous_as_rhs leaves out).
+void oracle_laplace_apply_data(int K, int N, const double* Dr, const double* Ds, const double* L,
+                               const double* cx, const double* cy, const int* nb_cell, const int* nb_face,
+                               int tau_mode, int neumann, const double* gd, const double* hd, double* u) {
   const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  const std::vector<int> boff = boundary_offsets(K, Nfp, nb_cell);
   std::vector<double> qx(size_t(K) * Np), qy(size_t(K) * Np), du(size_t(K) * 3 * Nfp);
   std::vector<double> Dx(Np * Np), Dy(Np * Np), tlx(3 * Nfp), tly(3 * Nfp);
   // pass 1: q = grad u with central flux (laplace_2d.h:145-197)
@@ -334,7 +351,7 @@ void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, c
       for (int i = 0; i < Nfp; ++i) {
         double d;
         if (isB)
-          d = neumann ? 0.0 : 2.0 * u[o + face_node(N, e, i)];
+          d = neumann ? 0.0 : 2.0 * u[o + face_node(N, e, i)] - (gd ? gd[boff[3 * c + e] + i] : 0.0);
         else
           d = u[o + face_node(N, e, i)] - u[size_t(nbc) * Np + face_node(N, nbf, Nfp - i - 1)];
         du[size_t(c) * 3 * Nfp + e * Nfp + i] = d;
@@ -367,7 +384,7 @@ void oracle_laplace_apply_bc(int K, int N, const double* Dr, const double* Ds, c
         const int a = face_node(N, e, i);
         double dqN;
         if (isB) {
-          dqN = neumann ? 2.0 * (qx[o + a] * nx + qy[o + a] * ny) : 0.0;
+          dqN = neumann ? 2.0 * (qx[o + a] * nx + qy[o + a] * ny) - (hd ? hd[boff[3 * c + e] + i] : 0.0) : 0.0;
         } else {
           const size_t b = size_t(nbc) * Np + face_node(N, nbf, Nfp - i - 1);
           dqN = (qx[o + a] - qx[b]) * nx + (qy[o + a] - qy[b]) * ny;

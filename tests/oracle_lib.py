@@ -40,6 +40,9 @@ def lib(threaded=False):
         L.oracle_laplace_apply_bc.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
                                               C.c_int, _dp]
         L.oracle_laplace_apply_bc.restype = None
+        L.oracle_laplace_apply_data.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
+                                                C.c_int, _dp, _dp, _dp]
+        L.oracle_laplace_apply_data.restype = None
         L.oracle_laplace_inhomogeneous.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
                                                    C.c_int, _dp, _dp, _dp]
         L.oracle_laplace_inhomogeneous.restype = None
@@ -113,6 +116,17 @@ class Problem:
         v = np.array(u, dtype=np.float64, copy=True)
         self._lib.oracle_laplace_apply_bc(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
                                           _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, neumann, _d(v))
+        return v
+
+    def laplace_data(self, u, neumann, gd=None, hd=None, tau_mode=0):
+        """The BR1 operator with boundary data (Dirichlet gd / Neumann hd per boundary node)."""
+        v = np.array(u, dtype=np.float64, copy=True)
+        gd = np.ascontiguousarray(gd, np.float64) if gd is not None else None
+        hd = np.ascontiguousarray(hd, np.float64) if hd is not None else None
+        self._lib.oracle_laplace_apply_data(self.K, self.N, _d(self.Dr), _d(self.Ds), _d(self.L), _d(self.cx),
+                                            _d(self.cy), _i(self.nb_cell), _i(self.nb_face), tau_mode, neumann,
+                                            _d(gd) if gd is not None else None,
+                                            _d(hd) if hd is not None else None, _d(v))
         return v
 
     def laplace_inhomogeneous(self, inout, neumann, gu=None, hq=None, tau_mode=0):

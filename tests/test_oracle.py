@@ -253,3 +253,69 @@ def test_neumann_laplacian_annihilates_constants():
         P = Problem.from_mesh(m, N, form="strong")
         assert np.abs(P.laplace_bc(np.ones(P.K * P.Np), 1)).max() < 1e-10
         assert np.abs(P.laplace_bc(np.ones(P.K * P.Np), 0)).max() > 1.0
+
+
+def test_laplace_data_without_data_is_the_homogeneous_operator():
+    m = dgc_amd.Mesh.structured(5, 4, -1, 1, -1, 1, jitter=0.1, seed=1)
+    P = Problem.from_mesh(m, 3, form="strong")
+    u = np.random.default_rng(3).standard_normal(P.K * P.Np)
+    for kind in (0, 1):
+        assert np.array_equal(P.laplace_data(u, kind), P.laplace_bc(u, kind))
+
+
+def _lap_errors(nxy, N=3):
+    """Relative L2 errors of Lap_h u against the exact Laplacian of a smooth u
+    with its Dirichlet data: the reference's operator()(u, bc) (homogeneous
+    part minus apply_inhomogeneous_as_rhs) and the whole BR1 operator."""
+    m = dgc_amd.Mesh.structured(nxy, nxy, -1, 1, -1, 1, jitter=0.1, seed=3)
+    P = Problem.from_mesh(m, N, form="strong")
+    x, y = m.dof_coords(N, node_major=False)
+    geo = m.geometry(N)
+    bx, by = geo["bx"], geo["by"]
+    u = np.sin(x) * np.cosh(0.5 * y)
+    lap = -0.75 * u
+    g = 2 * np.sin(bx) * np.cosh(0.5 * by)
+    ref = P.laplace_bc(u, 0) - P.laplace_inhomogeneous(np.zeros_like(u), 0, g, None)
+    full = P.laplace_data(u, 0, g, None)
+    return (np.linalg.norm(ref - lap) / np.linalg.norm(lap), np.linalg.norm(full - lap) / np.linalg.norm(lap))
+
+
+def test_laplace_with_dirichlet_data_is_consistent_only_with_the_whole_affine_part():
+    """The closed form pinning oracle_laplace_apply_data: the error of the whole
+    BR1 operator with Dirichlet data falls as the mesh is refined, while the
+    reference's operator()(u, bc), whose inhomogeneous part leaves out the
+    interior faces of boundary cells, does not converge."""
+    e4, e8, e16 = _lap_errors(4), _lap_errors(8), _lap_errors(16)
+    assert e8[1] < e4[1] / 2.5 and e16[1] < e8[1] / 2.5 and e16[1] < 0.05
+    assert e16[0] > e8[0] > e4[0] > 1.0
+
+
+def test_stokes_oracle_converges_to_the_exact_solution():
+    """The restated dual-splitting step (oracle/stokes_oracle.py) on the
+    example's analytic problem: five steps of dt = 1e-3 from the exact data at
+    t = 0 (the example's start: u^{n-1} = u^n), mean-square error as the
+    example reports it, on 4 x 4 and 8 x 8 jittered meshes at N = 2."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import stokes_oracle as so
+
+    errs = []
+    for nxy in (4, 8):
+        m = dgc_amd.Mesh.structured(nxy, nxy, -1, 1, -1, 1, jitter=0.1, seed=3)
+        N = 2
+        ow, os_ = dgc_amd.ref_operators(N, "weak"), dgc_amd.ref_operators(N, "strong")
+        Pw, Ps = Problem.from_mesh(m, N, ops=ow), Problem.from_mesh(m, N, ops=os_)
+        c = m.cells()
+        geo = so.Geometry(c["cx"], c["cy"], c["nb_cell"], os_["r"], os_["s"], os_["fmask"])
+        x, y = m.dof_coords(N, node_major=False)
+        up = u = so.exact(x, y, 0.0)
+        tp = t = 0.0
+        dt = 1e-3
+        for _ in range(5):
+            un = so.advance_timestep(Pw, Ps, geo, os_["M"], up, tp, u, t, dt)
+            up, tp, u, t = u, t, un, t + dt
+        ex = so.exact(x, y, t)
+        errs.append((np.sum((u[0] - ex[0]) ** 2) + np.sum((u[1] - ex[1]) ** 2)) / len(x))
+    assert errs[0] < 5e-3 and errs[1] < errs[0] / 10
