@@ -518,6 +518,47 @@ def graddiv_bench(dgc, ctx, N=3, reps=30):
     return out
 
 
+def stokes_bench(dgc, ctx):
+    """The Stokes example's run on the GPU (examples/stokes_2d/main.cpp:65-196):
+    the reference's fine mesh (tests/golden/square_domain_fine.msh, K = 1064),
+    dt = 1e-3 to T = 0.1 from u^{n-1} = u^n = exact, at N = 1 (the example's
+    default) and N = 2; wall time of the 100 steps, mean-square error against the
+    exact solution as the example prints it, iterations of the last step. Then
+    the step time on 128 x 128 squares (K = 32,768) at N = 2 over 5 steps."""
+    from dgc_amd.stokes import exact_velocity
+
+    out = {}
+    fine = dgc.Mesh.read_gmsh(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                            "square_domain_fine.msh"))
+    for name, mesh, N, steps in (("fine_N1", fine, 1, 100), ("fine_N2", fine, 2, 100),
+                                 ("K32768_N2", dgc.Mesh.structured(128, 128, -1, 1, -1, 1, jitter=0.05, seed=1), 2, 5)):
+        disc = dgc.Discretization(ctx, mesh, N)
+        st = dgc.Stokes(disc)
+        x, y = mesh.dof_coords(N, node_major=True)
+        ux, uy = exact_velocity(x, y, 0.0)
+        up = (dgc.DeviceArray.from_host(ctx, ux), dgc.DeviceArray.from_host(ctx, uy))
+        u = (dgc.DeviceArray.from_host(ctx, ux), dgc.DeviceArray.from_host(ctx, uy))
+        nxt = (dgc.DeviceArray(ctx, disc.size), dgc.DeviceArray(ctx, disc.size))
+        tp = t = 0.0
+        dt = 1e-3
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            its = st.advance(up, tp, u, t, dt, nxt, 1e-11)
+            up, u, nxt = u, nxt, up
+            tp, t = t, t + dt
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ex, ey = exact_velocity(x, y, t)
+        err = float((np.sum((u[0].download() - ex) ** 2) + np.sum((u[1].download() - ey) ** 2)) / len(x))
+        out[name] = dict(K=disc.K, N=N, steps=steps, T=round(t, 6), wall_s=round(wall, 4),
+                         ms_per_step=round(wall * 1e3 / steps, 3), error_norm=err, last_step_iterations=its)
+        print(f"# stokes {name}: {out[name]['ms_per_step']} ms/step, error norm {err:.3e}, iterations {its}",
+              file=sys.stderr, flush=True)
+        del st, disc
+    return out
+
+
 def maxwell_fp32_form(dgc, ctx, steps=50, warmup=5):
     """Maxwell TM on the sweep's 256 x 256 periodic lattice (K = 131072),
     N = 4..7, Ez = sin(pi x) sin(pi y): the FP64 path (autotuned tiling)
@@ -605,7 +646,7 @@ def main():
     ap.add_argument("--no-ipdg", action="store_true", help="skip the IPDG solver comparison")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 matrix-core form of the sweep")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched Maxwell ensemble")
-    ap.add_argument("--no-graddiv", action="store_true", help="skip the gradient / divergence timing")
+    ap.add_argument("--no-graddiv", action="store_true", help="skip the gradient / divergence and Stokes timings")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ipdg-full", action="store_true",
                     help="IPDG comparison at full size: 4096 training pairs, 256 held-out right-hand sides")
@@ -674,6 +715,7 @@ def main():
         extra["maxwell_fp32_mfma"] = maxwell_fp32_form(dgc, ctx)
     if not args.no_graddiv:
         extra["graddiv"] = graddiv_bench(dgc, ctx)
+        extra["stokes"] = stokes_bench(dgc, ctx)
 
     if not args.no_ipdg:
         full = args.ipdg_full

@@ -1,6 +1,8 @@
 // C ABI of the host set-up: reference-element operators, meshes, geometry.
 #include <cstring>
 #include <exception>
+#include <fstream>
+#include <limits>
 #include <memory>
 #include <string>
 
@@ -157,6 +159,31 @@ int dgc_mesh_dof_coords(dgc_mesh_t mesh, int order, int node_major, double* x, d
     std::memcpy(y, ys.data(), ys.size() * sizeof(double));
     return DGC_OK;
   });
+}
+
+int dgc_write_plot_file(const char* path, size_t n, const double* x, const double* y, const double* u,
+                        const double* ref) {
+  if (!path || !x || !u || !ref) return dgc_set_error(DGC_ERR_INVALID, "dgc_write_plot_file: null argument");
+  std::ofstream file(path);
+  if (!file) return dgc_set_error(DGC_ERR_INVALID, std::string("dgc_write_plot_file: cannot open ") + path);
+  file.precision(std::numeric_limits<double>::digits10);
+  if (y) {
+    // examples/advection_2d/main.cpp:164-174
+    file << "#         x         y         u" << std::endl;
+    for (size_t i = 0; i < n; ++i) file << x[i] << "  " << y[i] << "  " << u[i] << std::endl;
+    file << std::endl;
+    file << "#         x         y         reference solution" << std::endl;
+    for (size_t i = 0; i < n; ++i) file << x[i] << "  " << y[i] << "  " << ref[i] << std::endl;
+  } else {
+    // examples/advection_1d/main.cpp:124-134 (its reference block has one space)
+    file << "#         x         y" << std::endl;
+    for (size_t i = 0; i < n; ++i) file << x[i] << "  " << u[i] << std::endl;
+    file << std::endl;
+    file << "#         x         reference solution" << std::endl;
+    for (size_t i = 0; i < n; ++i) file << x[i] << " " << ref[i] << std::endl;
+  }
+  if (!file) return dgc_set_error(DGC_ERR_INVALID, std::string("dgc_write_plot_file: write failed: ") + path);
+  return DGC_OK;
 }
 
 }  // extern "C"

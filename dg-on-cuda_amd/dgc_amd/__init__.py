@@ -21,5 +21,6 @@ from .core import (  # noqa: F401
 from .ipdg import IPDG, poisson_rhs  # noqa: F401
 from .graddiv import GradDiv  # noqa: F401
 from .stokes import Stokes  # noqa: F401
+from .advection1d import Advection1D, write_plot_file  # noqa: F401
 from .mpnn import LearnedPreconditioner  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401

@@ -64,6 +64,12 @@ def _declare(L):
     d("dgc_grad2d_apply", [_vp, _vp, _vp, C.c_double, _vp, _vp])
     d("dgc_div2d_apply", [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
     d("dgc_stokes_create", [_vp, C.POINTER(_vp)])
+    d("dgc_adv1d_operators", [C.c_int, C.c_int, _dp, _dp, _dp])
+    d("dgc_adv1d_create", [_vp, C.c_int, C.c_int, _dp, _dp, C.POINTER(_vp)])
+    d("dgc_adv1d_destroy", [_vp])
+    d("dgc_adv1d_rhs", [_vp, _vp, C.c_double, _vp])
+    d("dgc_adv1d_rk4_steps", [_vp, _vp, C.c_double, C.c_double, C.c_int])
+    d("dgc_write_plot_file", [C.c_char_p, C.c_size_t, _dp, _dp, _dp, _dp])
     d("dgc_stokes_destroy", [_vp])
     d("dgc_stokes_advance", [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp,
                              _ip])

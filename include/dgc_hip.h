@@ -39,6 +39,7 @@ typedef struct dgc_ctx_s* dgc_ctx_t;
 typedef struct dgc_disc_s* dgc_disc_t;
 typedef struct dgc_graddiv_s* dgc_graddiv_t;
 typedef struct dgc_stokes_s* dgc_stokes_t;
+typedef struct dgc_adv1d_s* dgc_adv1d_t;
 typedef struct dgc_op_s* dgc_op_t;
 typedef struct dgc_mesh_s* dgc_mesh_t;
 
@@ -170,6 +171,34 @@ int dgc_stokes_destroy(dgc_stokes_t s);
 int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_prev, double t_prev,
                        const double* ux, const double* uy, double t, double dt, double rtol,
                        double* ux_out, double* uy_out, int* iters);
+
+/* ---- 1D advection (examples/advection_1d) --------------------------------
+ * advection_1d (examples/advection_1d/advection_1d.h:38-199) and its device
+ * object (d_advection_1d.cuh:37-98): u_t + 2 pi u_x = 0 on [0, 2 pi] with
+ * num_cells uniform cells, inflow -sin(2 pi t), upwind flux.
+ * dgc_adv1d_operators: the strong-form D = Dr 2/h and L = V V^T 2/h
+ * ((order+1)^2, row-major, the host class's m_D and m_L) and the node
+ * positions x [num_cells (order+1)] node-major; any output may be NULL.
+ * dgc_adv1d_create takes D and L (e.g. the reference's m_D, m_L) or builds
+ * them when either is NULL; orders 1..12. */
+int dgc_adv1d_operators(int num_cells, int order, double* D, double* L, double* x);
+int dgc_adv1d_create(dgc_ctx_t ctx, int num_cells, int order, const double* D, const double* L,
+                     dgc_adv1d_t* out);
+int dgc_adv1d_destroy(dgc_adv1d_t op);
+/* out = the right-hand side at time t (advection_1d::operator()); node-major, out != in */
+int dgc_adv1d_rhs(dgc_adv1d_t op, const double* in, double t, double* out);
+/* nsteps classical RK4 steps of dt from t0, in place (the loop of
+ * examples/advection_1d/main.cpp:105-116 with rk4_on_device), fused stages */
+int dgc_adv1d_rk4_steps(dgc_adv1d_t op, double* u, double t0, double dt, int nsteps);
+
+/* ---- plot data files -------------------------------------------------------
+ * The text files the examples write for plotting (examples/advection_2d/
+ * main.cpp:164-174 "Advection2DDataFile.txt", examples/advection_1d/
+ * main.cpp:124-134 "Advection1DDataFile.txt"): 15 significant digits, the
+ * solution block then the reference-solution block, with the examples'
+ * headers. y == NULL writes the 1D layout. Host arrays of n values. */
+int dgc_write_plot_file(const char* path, size_t n, const double* x, const double* y, const double* u,
+                        const double* ref);
 
 /* ---- semi-discrete operators (device objects) ----------------------------
  * Replace create_device_object / destroy_device_object of

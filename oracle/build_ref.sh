@@ -17,7 +17,7 @@ mv "$here/liboracle_omp.so.tmp" "$here/liboracle_omp.so"
 
 if [[ -d "$ref/src" ]]; then
   mkdir -p "$here/_ref"
-  inc=(-I"$ref/src" -I"$ref/src/mesh" -I"$ref/examples/maxwell_2d" -I"$ref/examples/advection_2d"
+  inc=(-I"$ref/src" -I"$ref/src/mesh" -I"$ref/examples/maxwell_2d" -I"$ref/examples/advection_2d" -I"$ref/examples/advection_1d"
        -I"$ref/examples/poisson_2d")
   # checked build (reference asserts on) for golden vectors
   "$HIPCC" -x hip --offload-arch=gfx950 -std=c++17 -O2 -w "${inc[@]}" \

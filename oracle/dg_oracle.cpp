@@ -569,6 +569,55 @@ void oracle_div(int K, int N, const double* Dr, const double* Ds, const double* 
   }
 }
 
+// 1D advection right-hand side: examples/advection_1d/advection_1d.h:155-197
+// (u_t + a u_x = 0, a = 2 pi, inflow -sin(2 pi t), outflow copy, upwind flux).
+// D, L: the strong-form m_D, m_L ((N+1)^2, row-major); in/out cell-major.
+void oracle_adv1d_rhs(int K, int N, const double* D, const double* L, const double* in, double t, double* out) {
+  const int np = N + 1;
+  const double a = 2.0 * M_PI;
+  auto flux = [a](double l, double r) { return a * 0.5 * (l + r) + a * 0.5 * (1.0 - 0.0) * (l - r); };
+  std::vector<double> F(K + 1), iv(np), ov(np);
+  for (int i = 0; i <= K; ++i) {
+    const double l = i > 0 ? in[i * np - 1] : -std::sin(2.0 * M_PI * t);
+    const double r = i < K ? in[i * np] : in[i * np - 1];
+    F[i] = flux(l, r);
+  }
+  for (int c = 0; c < K; ++c) {
+    gemv(D, np, np, -a, in + c * np, 0.0, out + c * np);
+    std::fill(iv.begin(), iv.end(), 0.0);
+    std::fill(ov.begin(), ov.end(), 0.0);
+    iv[0] = F[c] - a * in[c * np];
+    iv[np - 1] = a * in[(c + 1) * np - 1] - F[c + 1];
+    gemv(L, np, np, 1.0, iv.data(), 1.0, ov.data());
+    for (int j = 0; j < np; ++j) out[c * np + j] += ov[j];
+  }
+}
+
+// nsteps of the RK4 of src/explicit_runge_kutta.h:33-51 (host axpy_n,
+// src/axpy.h:112-122) on the 1D advection operator, in place from t0.
+void oracle_adv1d_rk4(int K, int N, const double* D, const double* L, double* u, double t0, double dt, int nsteps) {
+  const size_t n = size_t(K) * (N + 1);
+  std::vector<double> w0(n), w1(n), w2(n), w3(n), w4(n);
+  auto axpy = [n](double a, const double* x, const double* y, double* o) {
+    for (size_t i = 0; i < n; ++i) o[i] = a * x[i] + y[i];
+  };
+  double t = t0;
+  for (int s = 0; s < nsteps; ++s) {
+    oracle_adv1d_rhs(K, N, D, L, u, t, w1.data());
+    axpy(0.5 * dt, w1.data(), u, w0.data());
+    oracle_adv1d_rhs(K, N, D, L, w0.data(), t + 0.5 * dt, w2.data());
+    axpy(0.5 * dt, w2.data(), u, w0.data());
+    oracle_adv1d_rhs(K, N, D, L, w0.data(), t + 0.5 * dt, w3.data());
+    axpy(dt, w3.data(), u, w0.data());
+    oracle_adv1d_rhs(K, N, D, L, w0.data(), t + dt, w4.data());
+    axpy(2.0, w2.data(), w1.data(), w0.data());
+    axpy(2.0, w3.data(), w4.data(), w1.data());
+    axpy(dt / 6.0, w0.data(), u, w2.data());
+    axpy(dt / 6.0, w1.data(), w2.data(), u);
+    t += dt;
+  }
+}
+
 // Classical RK4 of src/explicit_runge_kutta.h:33-51 with the host axpy_n of
 // src/axpy.h:112-122, for either physics (phys 0: Maxwell, 1: advection).
 // u: F*K*Np cell-major, advanced in place over `nsteps` steps of dt from t0;

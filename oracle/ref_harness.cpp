@@ -23,6 +23,7 @@
 #include <thrust/host_vector.h>
 #include <thrust/iterator/zip_iterator.h>
 
+#include "advection_1d.h"
 #include "advection_2d.h"
 #include "axpy.h"
 #include "div_2d.h"
@@ -391,6 +392,38 @@ int main(int argc, char** argv) {
       op(v.begin(), bc);
       write(f, v);
     }
+    return 0;
+  }
+
+  if (cmd == "adv1d" && argc == 6) {
+    // examples/advection_1d: advection_1d<double>(numCells, order) and the
+    // loop of its main.cpp:105-116 (rk4 with axpy_n, dt = 0.25 / N^2 h / a).
+    // args: numCells order nsteps out. out: m_D, m_L (row-major), x, the
+    // initial state, its right-hand side at t = 0.3, the state after nsteps
+    // (CPU layout, cell-major).
+    const int K = std::atoi(argv[2]);
+    const int N = std::atoi(argv[3]);
+    const int nsteps = std::atoi(argv[4]);
+    advection_1d<double> op(K, N);
+    const int n = op.num_dofs();
+    std::vector<double> x(n), v(n), r(n), w1(n), w2(n), w3(n), w4(n), w5(n);
+    op.initialize_dofs(x.begin(), v.begin());
+    std::vector<double> v0 = v;
+    op(v0.cbegin(), n, 0.3, r.begin());
+    double t = 0.0;
+    const double dt = 0.25 / N / N * op.min_elem_size() / op.wave_speed();
+    for (int i = 0; i < nsteps; ++i) {
+      dgc::rk4(v.data(), n, t, dt, op, &dgc::axpy_n<double, const double*, double*>, w1.data(), w2.data(),
+               w3.data(), w4.data(), w5.data());
+      t += dt;
+    }
+    std::ofstream f(argv[5], std::ios::binary);
+    write_mat(f, op.m_D);
+    write_mat(f, op.m_L);
+    write(f, x);
+    write(f, v0);
+    write(f, r);
+    write(f, v);
     return 0;
   }
 

@@ -137,6 +137,7 @@ def main():
                         coarse_order=2, coarse_A=A)
     graddiv(tmp)
     laplace_bc(tmp)
+    adv1d(tmp)
     print("golden vectors written to", HERE)
 
 
@@ -196,10 +197,29 @@ def laplace_bc(tmp):
     np.savez_compressed(os.path.join(HERE, "laplace_bc_ref.npz"), **out)
 
 
+def adv1d(tmp):
+    """7. examples/advection_1d: (K, N, steps) = (16, 3, 50) and (64, 6, 20):
+    the operators, node positions, initial state, its right-hand side at
+    t = 0.3 and the state after the steps (dt = 0.25 / N^2 h / a)."""
+    out = {}
+    for K, N, steps in ((16, 3, 50), (64, 6, 20)):
+        opath = os.path.join(tmp, "a1.bin")
+        run("adv1d", K, N, steps, opath)
+        Np, n = N + 1, K * (N + 1)
+        r = np.fromfile(opath)
+        D, L, rest = r[:Np * Np], r[Np * Np:2 * Np * Np], r[2 * Np * Np:].reshape(4, n)
+        pre = f"K{K}_N{N}_"
+        out.update({pre + "D": D.reshape(Np, Np), pre + "L": L.reshape(Np, Np), pre + "x": rest[0],
+                    pre + "ic": rest[1], pre + "rhs03": rest[2], pre + "final": rest[3], pre + "steps": steps})
+    np.savez_compressed(os.path.join(HERE, "adv1d_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["graddiv"]:
         graddiv(tempfile.mkdtemp(prefix="dgc_golden_"))
     elif sys.argv[1:] == ["laplace_bc"]:
         laplace_bc(tempfile.mkdtemp(prefix="dgc_golden_"))
+    elif sys.argv[1:] == ["adv1d"]:
+        adv1d(tempfile.mkdtemp(prefix="dgc_golden_"))
     else:
         main()

@@ -46,6 +46,10 @@ def lib(threaded=False):
         L.oracle_laplace_inhomogeneous.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
                                                    C.c_int, _dp, _dp, _dp]
         L.oracle_laplace_inhomogeneous.restype = None
+        L.oracle_adv1d_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+        L.oracle_adv1d_rhs.restype = None
+        L.oracle_adv1d_rk4.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int]
+        L.oracle_adv1d_rk4.restype = None
         L.oracle_grad.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp,
                                   _dp, _dp]
         L.oracle_grad.restype = None
@@ -167,3 +171,21 @@ class Problem:
                                  _d(self.cx), _d(self.cy), _i(self.nb_cell), _i(self.nb_face),
                                  _d(self.r), _d(self.s), ax, ay, bc, _d(v), t0, dt, nsteps, t_end)
         return v, steps
+
+
+def adv1d_rhs(K, N, D, L, u, t):
+    """advection_1d::operator() restated (cell-major)."""
+    D = np.ascontiguousarray(D, np.float64)
+    L = np.ascontiguousarray(L, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros_like(u)
+    lib().oracle_adv1d_rhs(K, N, _d(D), _d(L), _d(u), t, _d(out))
+    return out
+
+
+def adv1d_rk4(K, N, D, L, u, t0, dt, nsteps):
+    D = np.ascontiguousarray(D, np.float64)
+    L = np.ascontiguousarray(L, np.float64)
+    v = np.array(u, dtype=np.float64, copy=True)
+    lib().oracle_adv1d_rk4(K, N, _d(D), _d(L), _d(v), t0, dt, nsteps)
+    return v

@@ -81,3 +81,13 @@ def test_laplace_driver_with_bc_objects(order):
     for kind, got in ((0, float(m.group(1))), (1, float(m.group(2)))):
         want = np.linalg.norm(P.laplace_bc(u, kind) - P.laplace_inhomogeneous(zero, kind, gu, hq))
         assert abs(got - want) <= 1e-12 * want
+
+
+def test_advection_1d_driver(tmp_path):
+    # the 1D example's flow (examples/advection_1d/main.cpp) with its plot file
+    plot = tmp_path / "Advection1DDataFile.txt"
+    out = run("advection_1d", 512, 4, 2000, plot)
+    m = re.search(r"error norm = ([0-9.e+-]+)", out)
+    assert m and float(m.group(1)) < 1e-12
+    lines = plot.read_text().splitlines()
+    assert lines[0] == "#         x         y" and lines[512 * 5 + 2] == "#         x         reference solution"
