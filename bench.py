@@ -524,16 +524,20 @@ def stokes_bench(dgc, ctx):
     dt = 1e-3 to T = 0.1 from u^{n-1} = u^n = exact, at N = 1 (the example's
     default) and N = 2; wall time of the 100 steps, mean-square error against the
     exact solution as the example prints it, iterations of the last step. Then
-    the step time on 128 x 128 squares (K = 32,768) at N = 2 over 5 steps."""
+    the step time on 128 x 128 squares (K = 32,768) at N = 2 over 5 steps, with
+    block-Jacobi and with two-level (32 x 32 boxes) pressure solves."""
     from dgc_amd.stokes import exact_velocity
 
     out = {}
     fine = dgc.Mesh.read_gmsh(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
                                             "square_domain_fine.msh"))
-    for name, mesh, N, steps in (("fine_N1", fine, 1, 100), ("fine_N2", fine, 2, 100),
-                                 ("K32768_N2", dgc.Mesh.structured(128, 128, -1, 1, -1, 1, jitter=0.05, seed=1), 2, 5)):
+    big = dgc.Mesh.structured(128, 128, -1, 1, -1, 1, jitter=0.05, seed=1)
+    for name, mesh, N, steps, coarse in (("fine_N1", fine, 1, 100, 0), ("fine_N2", fine, 2, 100, 0),
+                                         ("K32768_N2", big, 2, 5, 0), ("K32768_N2_two_level", big, 2, 5, 32)):
         disc = dgc.Discretization(ctx, mesh, N)
         st = dgc.Stokes(disc)
+        if coarse:
+            st.coarse_space(mesh, coarse)
         x, y = mesh.dof_coords(N, node_major=True)
         ux, uy = exact_velocity(x, y, 0.0)
         up = (dgc.DeviceArray.from_host(ctx, ux), dgc.DeviceArray.from_host(ctx, uy))

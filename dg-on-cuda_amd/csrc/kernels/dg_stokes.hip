@@ -201,6 +201,7 @@ struct dgc_stokes_s {
   double *bx = nullptr, *by = nullptr, *gtx = nullptr, *gty = nullptr, *gux = nullptr, *guy = nullptr, *hq = nullptr;
   double* Binv = nullptr;  // inverse blocks of -M Lap_h + sigma M (Dirichlet)
   double binv_sigma = -1.0;
+  int pressure_precond = 1;  // 1: block-Jacobi, 2: two-level (after dgc_stokes_coarse_space)
   std::vector<double> sblocks;  // diagonal blocks of -M Lap_h (Dirichlet), host
 };
 
@@ -350,6 +351,14 @@ int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out) {
   return DGC_OK;
 }
 
+int dgc_stokes_coarse_space(dgc_stokes_t s, const double* cell_x, const double* cell_y, int gx, int gy) {
+  DGC_REQUIRE(s != nullptr, "dgc_stokes_coarse_space: null solver");
+  int nc = 0;
+  DGC_CALL(dgc_ipdg_coarse_space(s->ip_p, cell_x, cell_y, gx, gy, &nc));
+  s->pressure_precond = 2;
+  return DGC_OK;
+}
+
 int dgc_stokes_destroy(dgc_stokes_t s) {
   if (s == nullptr) return DGC_OK;
   (void)hipStreamSynchronize(s->disc->ctx->stream);
@@ -400,7 +409,7 @@ int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_p
   {
     double rel = 0;
     float ms = 0;
-    DGC_CALL(dgc_ipdg_pcg(s->ip_p, s->b, s->p, rtol, maxit, 1, &it_p, &rel, nullptr, &ms));
+    DGC_CALL(dgc_ipdg_pcg(s->ip_p, s->b, s->p, rtol, maxit, s->pressure_precond, &it_p, &rel, nullptr, &ms));
   }
   // 3. projection: u~ -= (2 dt / 3) grad p, pressure BC exterior = interior (stokes_2d.h:340-349)
   DGC_CALL(dgc_grad2d_apply(s->gd, s->p, nullptr, 1.0, s->gx, s->gy));

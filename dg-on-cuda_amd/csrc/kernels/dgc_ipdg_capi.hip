@@ -687,10 +687,6 @@ extern "C" {
 int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell_y, int gx, int gy,
                           int* num_coarse) {
   DGC_REQUIRE(s && cell_x && cell_y, "dgc_ipdg_coarse_space: null argument");
-  if (s->neumann)
-    return dgc_set_error(DGC_ERR_UNSUPPORTED,
-                         "dgc_ipdg_coarse_space: the coarse space needs Dirichlet boundaries (with Neumann "
-                         "boundaries its coarse matrix is singular)");
   DGC_REQUIRE(gx >= 1 && gy >= 1, "dgc_ipdg_coarse_space: need gx, gy >= 1");
   dgc_disc_s* d = s->disc;
   const int K = d->K, Np = d->Np, N = d->order;
@@ -805,6 +801,62 @@ int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell
   if (e == hipSuccess) {
     s->coarse_A.resize(size_t(nc) * nc);
     e = hipMemcpyAsync(s->coarse_A.data(), A, s->coarse_A.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess && s->neumann) {
+    // With Neumann faces everywhere the constants are in the null space of S,
+    // and so is their coarse representation z (Phi z = 1, exact: every
+    // aggregate carries the constants). Solve for z aggregate by aggregate
+    // from the Gram blocks Phi_a^T Phi_a z_a = Phi_a^T 1, then invert
+    // A + alpha z z^T: for the PCG residuals, orthogonal to the constants,
+    // the coarse right-hand side is orthogonal to z and the correction is
+    // the one of A's pseudo-inverse.
+    std::vector<double> G(size_t(nc) * 3), rhs(nc);  // block diagonal: 3 entries per column
+    e = hipMemsetAsync(u, 0, n * sizeof(double), st);
+    std::vector<double> ones(n, 1.0);
+    if (e == hipSuccess) e = hipMemcpyAsync(u, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = dgc::coarse_restrict(N, cs, u, A, 1, nullptr, st);  // Phi^T 1 into A's first row
+    std::vector<double> col(size_t(nc) * nc);
+    if (e == hipSuccess) e = hipMemcpyAsync(rhs.data(), A, nc * sizeof(double), hipMemcpyDeviceToHost, st);
+    for (int c = 0; c < nc && e == hipSuccess; ++c) {
+      e = hipMemsetAsync(u, 0, n * sizeof(double), st);
+      if (e == hipSuccess) e = dgc::coarse_fill(N, cs, c, u, st);
+      if (e == hipSuccess) e = dgc::coarse_restrict(N, cs, u, A + c, nc, nullptr, st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(col.data(), A, col.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    std::vector<double> z(nc, 0.0);
+    for (int a = 0; e == hipSuccess && a < nc / 3; ++a) {
+      double g[3][4];
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) g[i][j] = col[size_t(3 * a + i) * nc + (3 * a + j)];
+        g[i][3] = rhs[3 * a + i];
+      }
+      for (int i = 0; i < 3; ++i) {  // Gauss-Jordan with partial pivoting on the 3 x 3 block
+        int piv = i;
+        for (int j = i + 1; j < 3; ++j)
+          if (std::abs(g[j][i]) > std::abs(g[piv][i])) piv = j;
+        for (int k = 0; k < 4; ++k) std::swap(g[i][k], g[piv][k]);
+        const double d = g[i][i];
+        for (int k = 0; k < 4; ++k) g[i][k] /= d;
+        for (int j = 0; j < 3; ++j)
+          if (j != i) {
+            const double f = g[j][i];
+            for (int k = 0; k < 4; ++k) g[j][k] -= f * g[i][k];
+          }
+      }
+      for (int i = 0; i < 3; ++i) z[3 * a + i] = g[i][3];
+    }
+    double tr = 0.0, zz = 0.0;
+    for (int i = 0; i < nc; ++i) {
+      tr += s->coarse_A[size_t(i) * nc + i];
+      zz += z[i] * z[i];
+    }
+    const double alpha = tr / nc / zz;
+    std::vector<double> areg(s->coarse_A);
+    for (int i = 0; i < nc; ++i)
+      for (int j = 0; j < nc; ++j) areg[size_t(i) * nc + j] += alpha * z[i] * z[j];
+    if (e == hipSuccess) e = hipMemcpyAsync(A, areg.data(), areg.size() * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
   if (e == hipSuccess) e = dgc::spd_inverse_inplace(A, nc, work, st);
   if (e == hipSuccess) e = dgc::symmetrize_to_float(A, nc, ld, dAinv, st);

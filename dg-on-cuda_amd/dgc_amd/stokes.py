@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .core import DeviceArray, Discretization, _i, _vp, check, lib
+from .core import DeviceArray, Discretization, _d, _i, _vp, check, lib
 
 A = 2.883356
 LAMBDA = 1 + A * A
@@ -26,6 +26,11 @@ class Stokes:
         h = _vp()
         check(lib().dgc_stokes_create(disc.handle, C.byref(h)))
         self._h = h
+
+    def coarse_space(self, mesh, gx: int, gy: int | None = None):
+        """Two-level pressure solves over gx x gy boxes of cells."""
+        c = mesh.cells()
+        check(lib().dgc_stokes_coarse_space(self._h, _d(c["cx"]), _d(c["cy"]), gx, gy if gy else gx))
 
     def advance(self, u_prev, t_prev, u, t, dt, out, rtol=1e-12) -> dict:
         """u_prev, u, out: pairs of DeviceArray (x, y components), node-major."""

@@ -167,6 +167,9 @@ int dgc_div2d_apply(dgc_graddiv_t op, const double* u0, const double* u1, const 
  * Velocities: device arrays, node-major [Np][K]. iters (may be NULL): the
  * pressure, x and y iteration counts. Synchronous. */
 int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out);
+/* Two-level pressure solves: the coarse space of dgc_ipdg_coarse_space on the
+ * pressure operator (cell corners cx, cy [3K] of the mesh, gx x gy boxes). */
+int dgc_stokes_coarse_space(dgc_stokes_t s, const double* cell_x, const double* cell_y, int gx, int gy);
 int dgc_stokes_destroy(dgc_stokes_t s);
 int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_prev, double t_prev,
                        const double* ux, const double* uy, double t, double dt, double rtol,
@@ -306,9 +309,10 @@ int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out);
  * src/laplace_2d.h:175,218 (default DGC_BC_DIRICHLET): Dirichlet faces take
  * u+ = -u-, q+ = q- in the homogeneous apply; Neumann faces u+ = u-,
  * q+.n = -q-.n (no penalty). Switching drops the block-Jacobi inverses and
- * the coarse space (rebuilt on demand; the coarse space is Dirichlet-only).
- * With Neumann faces everywhere the operator is singular (constants): CG and
- * block-Jacobi PCG need a right-hand side orthogonal to the constants. */
+ * the coarse space (rebuilt on demand). With Neumann faces everywhere the
+ * operator is singular (constants): the solvers need a right-hand side
+ * orthogonal to the constants; the coarse space then inverts its Galerkin
+ * matrix plus a rank-one term along the coarse constants. */
 enum { DGC_BC_DIRICHLET = 0, DGC_BC_NEUMANN = 1 };
 int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind);
 /* inout += the inhomogeneous boundary part of the Laplacian

@@ -46,9 +46,11 @@ def test_laplace_with_boundary_data_matches_oracle(ctx, N):
         assert rel(n2c(d.download(), K, Np), want) < 1e-12
 
 
-def _run_gpu(ctx, mesh, N, dt, steps, rtol=1e-13):
+def _run_gpu(ctx, mesh, N, dt, steps, rtol=1e-13, coarse=None):
     disc = dgc_amd.Discretization(ctx, mesh, N)
     st = dgc_amd.Stokes(disc)
+    if coarse:
+        st.coarse_space(mesh, coarse)
     x, y = mesh.dof_coords(N, node_major=True)
     ux, uy = so.exact(x, y, 0.0)
     up = (dgc_amd.DeviceArray.from_host(ctx, ux), dgc_amd.DeviceArray.from_host(ctx, uy))
@@ -96,3 +98,15 @@ def test_stokes_example_on_the_fine_mesh(ctx):
         print(f"N={N}: T={t:.3f} error norm {errs[-1]:.3e}, last step iterations {its[-1]}")
     assert abs(t - 0.1) < 1e-12
     assert errs[0] < 1e-3 and errs[1] < errs[0] / 5
+
+
+def test_stokes_with_two_level_pressure_solves(ctx):
+    """The same steps with the two-level pressure preconditioner: the same
+    velocities (to the solver tolerance) in fewer pressure iterations."""
+    mesh = dgc_amd.Mesh.structured(32, 32, -1, 1, -1, 1, jitter=0.1, seed=3)
+    _, ax, ay, _, its1 = _run_gpu(ctx, mesh, 2, 1e-3, 3, rtol=1e-12)
+    _, bx, by, _, its2 = _run_gpu(ctx, mesh, 2, 1e-3, 3, rtol=1e-12, coarse=8)
+    print("block-Jacobi:", its1, "two-level:", its2)
+    assert rel(bx, ax) < 1e-8 and rel(by, ay) < 1e-8
+    assert its2[-1]["pressure"] < its1[-1]["pressure"] / 2
+
