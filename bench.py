@@ -563,6 +563,33 @@ def stokes_bench(dgc, ctx):
     return out
 
 
+def adv1d_bench(dgc, ctx):
+    """The 1D example's run (examples/advection_1d/main.cpp:83-116): 8192 cells,
+    order 6, 10000 RK4 steps of dt = 0.25 / N^2 h / a from sin(x); wall time,
+    mean-square error against sin(x - 2 pi t) as the example prints it, and the
+    stage kernels' bandwidth: per cell and stage 8 (N+1) x (reads of the stage
+    input, u, and the RK streams; writes of the outputs): stages 1-3 move 4
+    vectors, stage 4 five."""
+    K, N, steps = 8192, 6, 10000
+    D, L, x = dgc.Advection1D.operators(K, N)
+    op = dgc.Advection1D(ctx, K, N)
+    u = dgc.DeviceArray.from_host(ctx, np.sin(x))
+    dt = 0.25 / N / N * (2 * np.pi / K) / (2 * np.pi)
+    op.rk4_steps(u, 0.0, dt, 10)  # warm-up (not part of the example's run)
+    u = dgc.DeviceArray.from_host(ctx, np.sin(x))
+    ctx.sync()
+    t0 = time.perf_counter()
+    op.rk4_steps(u, 0.0, dt, steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    err = float(np.mean((u.download() - np.sin(x - 2 * np.pi * steps * dt)) ** 2))
+    out = dict(K=K, N=N, steps=steps, wall_ms=round(wall * 1e3, 2), us_per_step=round(wall * 1e6 / steps, 2),
+               error_norm=err, dof_stage_per_s=K * (N + 1) * 4 * steps / wall)
+    print(f"# advection 1D K={K} N={N}: {steps} steps in {out['wall_ms']} ms ({out['us_per_step']} us/step), "
+          f"error norm {err:.3e}", file=sys.stderr, flush=True)
+    return out
+
+
 def maxwell_fp32_form(dgc, ctx, steps=50, warmup=5):
     """Maxwell TM on the sweep's 256 x 256 periodic lattice (K = 131072),
     N = 4..7, Ez = sin(pi x) sin(pi y): the FP64 path (autotuned tiling)
@@ -650,7 +677,13 @@ def main():
     ap.add_argument("--no-ipdg", action="store_true", help="skip the IPDG solver comparison")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 matrix-core form of the sweep")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched Maxwell ensemble")
-    ap.add_argument("--no-graddiv", action="store_true", help="skip the gradient / divergence and Stokes timings")
+    ap.add_argument("--no-graddiv", action="store_true",
+                    help="skip the gradient / divergence, Stokes and 1D advection timings")
+    ap.add_argument("--no-stokes", action="store_true",
+                    help="with --widened-only: skip the Stokes run (its graph-replayed pressure solves cannot be "
+                         "kernel-traced)")
+    ap.add_argument("--widened-only", action="store_true",
+                    help="run only the gradient / divergence, Stokes and 1D advection timings (for kernel traces)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ipdg-full", action="store_true",
                     help="IPDG comparison at full size: 4096 training pairs, 256 held-out right-hand sides")
@@ -673,6 +706,13 @@ def main():
         ctx.sync()
         print("# calibration: 3 x dgc_axpy over 2^27 doubles done", file=sys.stderr, flush=True)
         del x, y, o
+    if args.widened_only:
+        out = dict(workload="gradient / divergence, Stokes step, 1D advection", graddiv=graddiv_bench(dgc, ctx),
+                   advection_1d=adv1d_bench(dgc, ctx))
+        if not args.no_stokes:
+            out["stokes"] = stokes_bench(dgc, ctx)
+        print(json.dumps(out))
+        return
     if args.ipdg_apply_only:
         print(json.dumps(dict(workload="ipdg spd apply N=3", ipdg_apply=ipdg_apply_bench(dgc, ctx))))
         return
@@ -720,6 +760,7 @@ def main():
     if not args.no_graddiv:
         extra["graddiv"] = graddiv_bench(dgc, ctx)
         extra["stokes"] = stokes_bench(dgc, ctx)
+        extra["advection_1d"] = adv1d_bench(dgc, ctx)
 
     if not args.no_ipdg:
         full = args.ipdg_full
