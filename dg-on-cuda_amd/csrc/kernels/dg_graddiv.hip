@@ -10,10 +10,11 @@
 // as the reference calls its BC object), which covers the affine boundary
 // conditions of the Stokes splitting (examples/stokes_2d/stokes_2d.h:100-183).
 //
-// One workgroup of four waves per tile of 64 consecutive cells (lane = cell),
-// as the RK stage kernels: the tile's node rows are staged in LDS, the waves
-// split the face nodes for the fluxes and the node rows for the volume and
-// lift products. Both operators are HBM-bound: per cell they read the input
+// WT waves per tile of 64 consecutive cells (lane = cell), 4 / WT tiles per
+// 256-thread workgroup, as the RK stage kernels: the tile's node rows are
+// staged in LDS, the tile's waves split the face nodes for the fluxes and the
+// node rows for the volume and lift products. Fewer waves per tile give each
+// wave more rows in flight. Both operators are HBM-bound: per cell they read the input
 // rows, four metric terms and three connectivity words and write the output
 // rows once.
 #include <hip/hip_runtime.h>
@@ -56,23 +57,34 @@ struct GradDivParams {
   double* out1;  // grad: y component
 };
 
+// LDS bytes of a workgroup: node rows and face fluxes of its 4 / WT tiles
+constexpr int graddiv_lds_bytes(int N, bool div, int wt) {
+  return (kWaves / wt) * ((div ? 2 : 1) * (N + 1) * (N + 2) / 2 + (div ? 1 : 2) * 3 * (N + 1)) * kTile * 8;
+}
+// a tiling is built when its LDS fits two workgroups per CU (WT = 4 always)
+constexpr bool graddiv_fits(int N, bool div, int wt) { return wt == 4 || graddiv_lds_bytes(N, div, wt) <= 64 * 1024; }
+
 // DIV = false: gradient of in0 into (out0, out1); true: divergence of (in0, in1) into out0.
-template <int N, bool DIV>
+template <int N, bool DIV, int WT>
 __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) {
   using T = Tri<N>;
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
   constexpr int FI = DIV ? 2 : 1;  // input fields
   constexpr int FO = DIV ? 1 : 2;  // flux / output fields
-  constexpr int RPW = (Np + kWaves - 1) / kWaves;
-  constexpr int FPW = (NF + kWaves - 1) / kWaves;
-  constexpr int S = kWaves * RPW;
-  __shared__ double us[FI][Np][kTile];
-  __shared__ double fl[FO][NF][kTile];
+  constexpr int TPW = kWaves / WT;  // tiles per workgroup
+  constexpr int RPW = (Np + WT - 1) / WT;
+  constexpr int FPW = (NF + WT - 1) / WT;
+  constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
+  __shared__ double lds_us[TPW][FI][Np][kTile];
+  __shared__ double lds_fl[TPW][FO][NF][kTile];
 
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
-  const int wt = uniform(threadIdx.x >> 6);
-  const int c0 = xcd_group(blockIdx.x, gridDim.x) * kTile;
+  const int wave = uniform(threadIdx.x >> 6);
+  const int tw = wave / WT, wt = wave - tw * WT;
+  auto& us = lds_us[tw];
+  auto& fl = lds_fl[tw];
+  const int c0 = (xcd_group(blockIdx.x, gridDim.x) * TPW + tw) * kTile;
   const int c = c0 + lane;
   const bool valid = c < K;
   const double* in[2] = {p.in0, p.in1};
@@ -82,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
   int code[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wt + kWaves * i;
+    const int m = wt + WT * i;
     code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
   }
   double rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0;
@@ -95,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
   double tl[FI][RPW];
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wt + kWaves * i;
+    const int j = wt + WT * i;
 #pragma unroll
     for (int f = 0; f < FI; ++f) tl[f][i] = (valid && j < Np) ? in[f][(size_t)j * K + c] : 0.0;
   }
@@ -103,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
   double gv[FI][FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int d = (wt + kWaves * i) % Nfp;
+    const int d = (wt + WT * i) % Nfp;
     const int nf = code[i] & 3, idx = code[i] >> 2;
     const bool far = nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile;
     const int nn = T::fnode(nf, Nfp - 1 - d);
@@ -112,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
   }
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const int j = wt + kWaves * i;
+    const int j = wt + WT * i;
     if (j < Np) {
 #pragma unroll
       for (int f = 0; f < FI; ++f) us[f][j][lane] = tl[f][i];
@@ -123,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
   // ---- 2. central fluxes at the face nodes, scaled by face_J / J
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wt + kWaves * i;
+    const int m = wt + WT * i;
     if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const int own = T::fnode(e, d);
@@ -224,16 +236,27 @@ __global__ __launch_bounds__(kBlock) void graddiv_kernel(const GradDivParams p) 
 
 using GradDivFn = void (*)(GradDivParams);
 
+template <int N, bool DIV>
+GradDivFn pick_wt(int wt) {
+  if constexpr (graddiv_fits(N, DIV, 1)) {
+    if (wt == 1) return graddiv_kernel<N, DIV, 1>;
+  }
+  if constexpr (graddiv_fits(N, DIV, 2)) {
+    if (wt == 2) return graddiv_kernel<N, DIV, 2>;
+  }
+  return wt == 4 ? graddiv_kernel<N, DIV, 4> : nullptr;
+}
+
 template <bool DIV>
-GradDivFn pick(int order) {
+GradDivFn pick(int order, int wt) {
   switch (order) {
-    case 1: return graddiv_kernel<1, DIV>;
-    case 2: return graddiv_kernel<2, DIV>;
-    case 3: return graddiv_kernel<3, DIV>;
-    case 4: return graddiv_kernel<4, DIV>;
-    case 5: return graddiv_kernel<5, DIV>;
-    case 6: return graddiv_kernel<6, DIV>;
-    case 7: return graddiv_kernel<7, DIV>;
+    case 1: return pick_wt<1, DIV>(wt);
+    case 2: return pick_wt<2, DIV>(wt);
+    case 3: return pick_wt<3, DIV>(wt);
+    case 4: return pick_wt<4, DIV>(wt);
+    case 5: return pick_wt<5, DIV>(wt);
+    case 6: return pick_wt<6, DIV>(wt);
+    case 7: return pick_wt<7, DIV>(wt);
     default: return nullptr;
   }
 }
@@ -244,6 +267,7 @@ GradDivFn pick(int order) {
 struct dgc_graddiv_s {
   dgc_disc_s* disc = nullptr;
   double *DrT = nullptr, *DsT = nullptr, *LT = nullptr;
+  int wt = 0;  // waves per tile; 0: the default for the order
 };
 
 extern "C" {
@@ -313,12 +337,30 @@ static int graddiv_launch(dgc_graddiv_t op, bool div, const double* u0, const do
   p.beta = beta;
   p.out0 = out0;
   p.out1 = out1;
-  dgc::GradDivFn fn = div ? dgc::pick<true>(d->order) : dgc::pick<false>(d->order);
+  // default tiling per order, measured at K = 2.1M (1 wave per tile at N <= 2:
+  // 6.2 / 3.8 TB/s against 3.6 / 2.9 with 4; 2 at N = 3 and 5; N >= 6 only
+  // fits 4 within 64 KB of LDS)
+  static constexpr int kDefaultWt[8] = {4, 1, 1, 2, 4, 2, 4, 4};
+  int wt = op->wt != 0 ? op->wt : kDefaultWt[d->order];
+  dgc::GradDivFn fn = div ? dgc::pick<true>(d->order, wt) : dgc::pick<false>(d->order, wt);
+  if (fn == nullptr) {  // the tiling is not built for this order: one tile per workgroup
+    wt = 4;
+    fn = div ? dgc::pick<true>(d->order, wt) : dgc::pick<false>(d->order, wt);
+  }
   if (fn == nullptr) return dgc_set_error(DGC_ERR_UNSUPPORTED, std::string(name) + ": order not built");
   DGC_HIP_TRY(hipSetDevice(d->ctx->device));
   const int tiles = (d->K + dgc::kTile - 1) / dgc::kTile;
-  hipLaunchKernelGGL(fn, dim3(tiles), dim3(dgc::kBlock), 0, d->ctx->stream, p);
+  const int tpw = dgc::kWaves / wt;
+  hipLaunchKernelGGL(fn, dim3((tiles + tpw - 1) / tpw), dim3(dgc::kBlock), 0, d->ctx->stream, p);
   DGC_HIP_TRY(hipGetLastError());
+  return DGC_OK;
+}
+
+int dgc_graddiv_set_tiling(dgc_graddiv_t op, int waves_per_tile) {
+  DGC_REQUIRE(op != nullptr, "dgc_graddiv_set_tiling: null operator");
+  DGC_REQUIRE(waves_per_tile == 0 || waves_per_tile == 1 || waves_per_tile == 2 || waves_per_tile == 4,
+              "dgc_graddiv_set_tiling: waves per tile must be 0 (default), 1, 2 or 4");
+  op->wt = waves_per_tile;
   return DGC_OK;
 }
 

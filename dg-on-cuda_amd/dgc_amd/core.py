@@ -61,6 +61,7 @@ def _declare(L):
     d("dgc_disc2d_arrays", [_vp, _dp, _ip, _dp, _dp, _ip, _dp, _dp])
     d("dgc_graddiv_create", [_vp, _dp, _dp, _dp, C.POINTER(_vp)])
     d("dgc_graddiv_destroy", [_vp])
+    d("dgc_graddiv_set_tiling", [_vp, C.c_int])
     d("dgc_grad2d_apply", [_vp, _vp, _vp, C.c_double, _vp, _vp])
     d("dgc_div2d_apply", [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
     d("dgc_stokes_create", [_vp, C.POINTER(_vp)])

@@ -32,6 +32,10 @@ class GradDiv:
                                        _d(np.ascontiguousarray(ops["LIFT"])), C.byref(h)))
         self._h = h
 
+    def set_tiling(self, waves_per_tile: int):
+        """1, 2 or 4 waves per 64-cell tile (0: the order's default)."""
+        check(lib().dgc_graddiv_set_tiling(self._h, waves_per_tile))
+
     def grad(self, u: DeviceArray, out_x: DeviceArray, out_y: DeviceArray, g: DeviceArray | None = None,
              beta: float = 1.0):
         check(lib().dgc_grad2d_apply(self._h, _vp(u.ptr), _ptr(g), beta, _vp(out_x.ptr), _vp(out_y.ptr)))

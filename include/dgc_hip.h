@@ -146,6 +146,12 @@ int dgc_disc2d_arrays(dgc_disc_t disc, double* geo, int* nbr, double* J, double*
 int dgc_graddiv_create(dgc_disc_t disc, const double* Dr, const double* Ds, const double* lift,
                        dgc_graddiv_t* out);
 int dgc_graddiv_destroy(dgc_graddiv_t op);
+/* Waves per 64-cell tile of the kernels: 1, 2 or 4 (4 / wt tiles per
+ * 256-thread workgroup; a tiling whose LDS would exceed 64 KB runs as 4), or 0
+ * for the default of the order (1 at N <= 2, 2 at N = 3, 5, else 4). The
+ * tilings agree to rounding (the compiler may contract the metric products
+ * differently in each). */
+int dgc_graddiv_set_tiling(dgc_graddiv_t op, int waves_per_tile);
 /* (out_x, out_y) = grad u  (grad_2d::operator(), src/grad_2d.h:152-187) */
 int dgc_grad2d_apply(dgc_graddiv_t op, const double* u, const double* g, double beta, double* out_x,
                      double* out_y);

@@ -109,3 +109,19 @@ def test_grad_div_reject_aliasing(ctx):
         c.op.grad(u, u, v)
     with pytest.raises(dgc_amd.DGCError, match="alias"):
         c.op.div(u, v, v)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5])
+def test_grad_div_tilings_agree(ctx, N):
+    mesh = dgc_amd.Mesh.structured(40, 30, -1, 1, -1, 1, jitter=0.15, seed=N)
+    c = Case(ctx, mesh, N)
+    rng = np.random.default_rng(400 + N)
+    u0, u1 = rng.standard_normal((2, c.K * c.Np))
+    g0, g1, beta = c.bcs()[1]
+    base = c.grad(u0, g0, beta) + (c.div(u0, u1, g0, g1, beta),)
+    for wt in (1, 2, 4):
+        c.op.set_tiling(wt)
+        got = c.grad(u0, g0, beta) + (c.div(u0, u1, g0, g1, beta),)
+        for a, b in zip(got, base):
+            assert rel(a, b) < 1e-14
+
