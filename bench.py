@@ -563,6 +563,34 @@ def stokes_bench(dgc, ctx):
     return out
 
 
+def assemble_bench(dgc, ctx, N=3):
+    """The IPDG Laplacian as a CSR matrix (dgc_ipdg_assemble) on 128 x 128 squares
+    (K = 32,768) at N = 3: wall time of the assembly (host lists + GPU probing +
+    download of the values) against the time the reference's unit-vector
+    assembly (examples/stokes_2d/stokes_2d.h:388-422) would take with this
+    library's own apply, i.e. n applications."""
+    mesh = dgc.Mesh.structured(128, 128, -1, 1, -1, 1, jitter=0.05, seed=1)
+    disc = dgc.Discretization(ctx, mesh, N)
+    ip = dgc.IPDG(disc, tau_mode=1)
+    ip.assemble()  # warm-up
+    t0 = time.perf_counter()
+    rp, ci, va = ip.assemble()
+    wall = time.perf_counter() - t0
+    u = dgc.DeviceArray.from_host(ctx, np.random.default_rng(0).standard_normal(disc.size))
+    for _ in range(5):
+        ip.apply(u)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        ip.apply(u)
+    ctx.sync()
+    apply_s = (time.perf_counter() - t0) / 50
+    out = dict(K=disc.K, N=N, n=disc.size, nnz=int(len(va)), wall_s=round(wall, 4),
+               apply_us=round(apply_s * 1e6, 2), unit_vector_estimate_s=round(apply_s * disc.size, 2))
+    print(f"# assemble: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def adv1d_bench(dgc, ctx):
     """The 1D example's run (examples/advection_1d/main.cpp:83-116): 8192 cells,
     order 6, 10000 RK4 steps of dt = 0.25 / N^2 h / a from sin(x); wall time,
@@ -707,8 +735,9 @@ def main():
         print("# calibration: 3 x dgc_axpy over 2^27 doubles done", file=sys.stderr, flush=True)
         del x, y, o
     if args.widened_only:
-        out = dict(workload="gradient / divergence, Stokes step, 1D advection", graddiv=graddiv_bench(dgc, ctx),
-                   advection_1d=adv1d_bench(dgc, ctx))
+        out = dict(workload="gradient / divergence, Stokes step, 1D advection, IPDG assembly",
+                   graddiv=graddiv_bench(dgc, ctx), advection_1d=adv1d_bench(dgc, ctx),
+                   assemble=assemble_bench(dgc, ctx))
         if not args.no_stokes:
             out["stokes"] = stokes_bench(dgc, ctx)
         print(json.dumps(out))
@@ -761,6 +790,7 @@ def main():
         extra["graddiv"] = graddiv_bench(dgc, ctx)
         extra["stokes"] = stokes_bench(dgc, ctx)
         extra["advection_1d"] = adv1d_bench(dgc, ctx)
+        extra["assemble"] = assemble_bench(dgc, ctx)
 
     if not args.no_ipdg:
         full = args.ipdg_full

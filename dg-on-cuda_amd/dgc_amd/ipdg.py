@@ -61,6 +61,19 @@ class IPDG:
         check(lib().dgc_ipdg_apply_data(self._h, _vp(gd.ptr) if gd is not None else None,
                                         _vp(hd.ptr) if hd is not None else None, _vp(u.ptr)))
 
+    def assemble(self):
+        """The operator of apply() as (row_ptr, col_idx, values), CSR over the
+        node-major DOF order (scipy.sparse.csr_matrix((values, col_idx, row_ptr)))."""
+        n = self.disc.size
+        nnz = C.c_longlong()
+        rp = np.zeros(n + 1, np.int64)
+        check(lib().dgc_ipdg_assemble(self._h, C.byref(nnz), rp.ctypes.data_as(C.POINTER(C.c_longlong)), None, None))
+        ci = np.zeros(nnz.value, np.int32)
+        va = np.zeros(nnz.value)
+        check(lib().dgc_ipdg_assemble(self._h, C.byref(nnz), rp.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                      ci.ctypes.data_as(C.POINTER(C.c_int)), _d(va)))
+        return rp, ci, va
+
     def mass(self, f: DeviceArray, out: DeviceArray):
         check(lib().dgc_ipdg_mass(self._h, _vp(f.ptr), _vp(out.ptr)))
 

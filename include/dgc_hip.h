@@ -339,6 +339,15 @@ int dgc_ipdg_apply_inhomogeneous(dgc_ipdg_t s, const double* gu, const double* h
  * operator()(u, bc) error grows as the mesh is refined). No counterpart in the
  * reference; the Stokes step uses it. */
 int dgc_ipdg_apply_data(dgc_ipdg_t s, const double* gd, const double* hd, double* u);
+/* The operator of dgc_ipdg_apply (current boundary kind) as a sparse matrix in
+ * CSR form, rows and columns in the node-major DOF order j K + c: the
+ * "implicit" laplace_2d of examples/stokes_2d/stokes_2d.h:74-76, which the
+ * reference assembles by one application per unit vector (stokes_2d.h:388-422);
+ * here by probing a colouring of the cells (tens of applications). Call once
+ * with col_idx == values == NULL for *nnz (and row_ptr [K Np + 1] if given),
+ * then with host arrays row_ptr, col_idx [nnz], values [nnz]. Row block c
+ * holds the columns of the cells within two faces of c (sorted), node by node. */
+int dgc_ipdg_assemble(dgc_ipdg_t s, long long* nnz, long long* row_ptr, int* col_idx, double* values);
 /* Builds the block-Jacobi preconditioner: the exact diagonal blocks of
  * -M Lap_h, probed with a distance-2 colouring of the cells, inverted per cell. */
 int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);
