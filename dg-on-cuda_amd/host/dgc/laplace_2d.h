@@ -75,6 +75,22 @@ class laplace_2d : public simple_discretization_2d<T, M> {
     check(dgc_axpy(m_ctx, -1.0, z.data(), z.size(), d_u, d_u), "axpy failed");
   }
 
+  // The homogeneous operator of the BC's kind as a CSR matrix over the
+  // node-major DOFs: the "implicit" laplace_2d of examples/stokes_2d/stokes_2d.h:74-76
+  // in place of the unit-vector loop of stokes_2d.h:388-422.
+  template <typename BC2D>
+  void assemble(const BC2D& bc, std::vector<long long>& row_ptr, std::vector<int>& col_idx,
+                std::vector<T>& values) const {
+    set_kind(bc);
+    long long nnz = 0;
+    row_ptr.assign(this->total_num_nodes() + 1, 0);
+    check(dgc_ipdg_assemble(m_op->handle(), &nnz, row_ptr.data(), nullptr, nullptr), "assembly sizes failed");
+    col_idx.resize(std::size_t(nnz));
+    values.resize(std::size_t(nnz));
+    check(dgc_ipdg_assemble(m_op->handle(), &nnz, row_ptr.data(), col_idx.data(), values.data()),
+          "assembly failed");
+  }
+
   // -Lap_h u = f: b = M f, then block-Jacobi PCG to the relative residual rtol.
   // Returns the iteration count.
   int solve(const T* d_f, T* d_u, double rtol = 1e-10, int maxit = 100000) const {

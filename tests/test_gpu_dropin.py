@@ -81,6 +81,8 @@ def test_laplace_driver_with_bc_objects(order):
     for kind, got in ((0, float(m.group(1))), (1, float(m.group(2)))):
         want = np.linalg.norm(P.laplace_bc(u, kind) - P.laplace_inhomogeneous(zero, kind, gu, hq))
         assert abs(got - want) <= 1e-12 * want
+    m = re.search(r"assembled: nnz = (\d+), max \|A u - Lap u\| / max \|Lap u\| = ([0-9.e+-]+)", out)
+    assert m and int(m.group(1)) > 0 and float(m.group(2)) < 1e-13
 
 
 def test_advection_1d_driver(tmp_path):
