@@ -371,7 +371,31 @@ __global__ __launch_bounds__(64 * W) void ipdg_pass2(const IpdgParams<T> p) {
     }
     return;
   }
-  // SPD form: out = -J (M Lap u), M already in the tables
+  // SPD form: out = -J (M Lap u), M already in the tables; with a shift,
+  // + shift J M u: the operand's rows replace q_x in LDS (every wave has read
+  // q_x above), then the mass rows are applied like the other tables
+  if (p.shift != T(0)) {
+    const T* sv = p.dirp != nullptr ? p.dirp : p.u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int j = wave + W * i;
+      if (j < Np) qxs[j][lane] = valid ? sv[(size_t)j * K + c] : T(0);
+    }
+    __syncthreads();
+    T ms[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) ms[i] = 0;
+#pragma unroll
+    for (int k = 0; k < Np; ++k) {
+      const T v = qxs[k][lane];
+      const auto* mm = table(p.MT) + k * Npad + j0;
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) ms[i] = fma(mm[i], v, ms[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) lap[i] = fma(-p.shift, ms[i], lap[i]);
+  }
   T dot = 0;
   if (valid) {
 #pragma unroll

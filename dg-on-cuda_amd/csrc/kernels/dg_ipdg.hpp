@@ -53,6 +53,10 @@ struct IpdgParams {
   // normal derivative at interior 0 on Neumann faces (q+.n = hbd - q-.n)
   const double* gbd;
   const double* hbd;
+  // SPD form only: out = -J M Lap u + shift J M u (0: no shift term). The
+  // shifted operand is the vector the operator is applied to: dirp when the
+  // search direction is formed on the fly, else u.
+  T shift;
   const T* u;
   T* qx;
   T* qy;

@@ -20,7 +20,9 @@
 // solver on the host (stokes_2d.h:388-526); here the
 // pressure solve is block-Jacobi PCG on -M Lap_h (singular with Neumann faces:
 // the right-hand side is projected onto the complement of the constants), and
-// the Helmholtz solves are block-Jacobi PCG on -M Lap_h + (3 / 2dt) M.
+// the Helmholtz solves are block-Jacobi PCG on the shifted SPD form
+// -M Lap_h + (3 / 2dt) M (dgc_ipdg_set_shift), both with the IPDG solver's
+// fused, graph-captured iterations.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -159,17 +161,6 @@ __global__ __launch_bounds__(256) void pressure_data_kernel(const dgc::IpdgParam
   }
 }
 
-// z = B r with one dense Np x Np block per cell: B[(j Np + k) K + c]
-__global__ void block_apply_kernel(int K, int Np, const double* B, const double* r, double* z) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= K) return;
-  for (int j = 0; j < Np; ++j) {
-    double v = 0;
-    for (int k = 0; k < Np; ++k) v += B[((size_t)j * Np + k) * K + c] * r[(size_t)k * K + c];
-    z[(size_t)j * K + c] = v;
-  }
-}
-
 hipError_t pressure_data(int order, const dgc::IpdgParams<double>& p, const double* ux, const double* uy,
                          const double* uxp, const double* uyp, const double* bx, const double* by, double t_next,
                          double* hq, hipStream_t s) {
@@ -194,15 +185,11 @@ struct dgc_stokes_s {
   dgc_graddiv_t gd = nullptr;
   dgc_ipdg_t ip_p = nullptr;  // Neumann faces (pressure)
   dgc_ipdg_t ip_u = nullptr;  // Dirichlet faces (velocity)
-  std::vector<double> mass;   // reference mass matrix
   std::vector<void*> owned;
   double *ut_x = nullptr, *ut_y = nullptr, *w = nullptr, *b = nullptr, *p = nullptr, *gx = nullptr, *gy = nullptr;
-  double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr, *tmp = nullptr, *ones = nullptr;
+  double *r = nullptr, *d = nullptr, *q = nullptr, *tmp = nullptr, *ones = nullptr;
   double *bx = nullptr, *by = nullptr, *gtx = nullptr, *gty = nullptr, *gux = nullptr, *guy = nullptr, *hq = nullptr;
-  double* Binv = nullptr;  // inverse blocks of -M Lap_h + sigma M (Dirichlet)
-  double binv_sigma = -1.0;
   int pressure_precond = 1;  // 1: block-Jacobi, 2: two-level (after dgc_stokes_coarse_space)
-  std::vector<double> sblocks;  // diagonal blocks of -M Lap_h (Dirichlet), host
 };
 
 namespace {
@@ -218,94 +205,30 @@ int salloc(dgc_stokes_s* s, T** p, size_t n) {
   return DGC_OK;
 }
 
-// Inverse blocks of -M Lap_h + sigma J M_ref per cell (Gauss-Jordan with
-// partial pivoting on the host), uploaded to s->Binv.
-int build_block_inverse(dgc_stokes_s* s, double sigma) {
-  const dgc_disc_s* d = s->disc;
-  const int K = d->K, Np = d->Np;
-  if (s->sblocks.empty()) {
-    DGC_CALL(dgc_ipdg_block_jacobi(s->ip_u, nullptr));
-    s->sblocks.resize(size_t(Np) * Np * K);
-    DGC_CALL(dgc_ipdg_diag_blocks(s->ip_u, s->sblocks.data()));
-  }
-  std::vector<double> inv(size_t(Np) * Np * K), a(size_t(Np) * 2 * Np);
-  for (int c = 0; c < K; ++c) {
-    for (int j = 0; j < Np; ++j)
-      for (int k = 0; k < 2 * Np; ++k)
-        a[j * 2 * Np + k] = k < Np ? s->sblocks[(size_t(j) * Np + k) * K + c] + sigma * d->h_J[c] * s->mass[j * Np + k]
-                                   : (k - Np == j ? 1.0 : 0.0);
-    for (int i = 0; i < Np; ++i) {
-      int piv = i;
-      for (int j = i + 1; j < Np; ++j)
-        if (std::abs(a[j * 2 * Np + i]) > std::abs(a[piv * 2 * Np + i])) piv = j;
-      if (a[piv * 2 * Np + i] == 0.0) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_stokes: singular diagonal block");
-      if (piv != i)
-        for (int k = 0; k < 2 * Np; ++k) std::swap(a[i * 2 * Np + k], a[piv * 2 * Np + k]);
-      const double dv = a[i * 2 * Np + i];
-      for (int k = 0; k < 2 * Np; ++k) a[i * 2 * Np + k] /= dv;
-      for (int j = 0; j < Np; ++j)
-        if (j != i) {
-          const double f = a[j * 2 * Np + i];
-          for (int k = 0; k < 2 * Np; ++k) a[j * 2 * Np + k] -= f * a[i * 2 * Np + k];
-        }
-    }
-    for (int j = 0; j < Np; ++j)
-      for (int k = 0; k < Np; ++k) inv[(size_t(j) * Np + k) * K + c] = a[j * 2 * Np + Np + k];
-  }
-  DGC_HIP_TRY(hipMemcpy(s->Binv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
-  s->binv_sigma = sigma;
-  return DGC_OK;
-}
-
-// q = (-M Lap_h + sigma M) v  (Dirichlet faces)
-int shifted_apply(dgc_stokes_s* s, double sigma, const double* v, double* q) {
-  hipStream_t st = s->disc->ctx->stream;
-  DGC_CALL(dgc_ipdg_apply_spd(s->ip_u, v, q));
-  DGC_CALL(dgc_ipdg_mass(s->ip_u, v, s->tmp));
-  hipLaunchKernelGGL(axpby_kernel, grid_of(n_of(s)), dim3(256), 0, st, sigma, (const double*)s->tmp, 1.0,
-                     (const double*)q, q, n_of(s));
-  DGC_HIP_TRY(hipGetLastError());
-  return DGC_OK;
-}
-
-// Block-Jacobi PCG on (-M Lap_h + sigma M) x = b from the initial guess in x.
+// (-M Lap_h + sigma M) x = b from the initial guess in x, to
+// |b - S x| <= rtol |b|: the shifted IPDG operator's block-Jacobi PCG (fused,
+// graph-captured iterations) solves for the correction e of r0 = b - S x0 to
+// |r0 - S e| <= (rtol |b| / |r0|) |r0|.
 int shifted_pcg(dgc_stokes_s* s, double sigma, const double* b, double* x, double rtol, int maxit, int* iters) {
   hipStream_t st = s->disc->ctx->stream;
   const size_t n = n_of(s);
-  const int K = s->disc->K, Np = s->disc->Np;
-  if (s->binv_sigma != sigma) DGC_CALL(build_block_inverse(s, sigma));
-  double bb = 0, rr = 0, rz = 0, pq = 0;
+  DGC_CALL(dgc_ipdg_set_shift(s->ip_u, sigma));
+  double bb = 0, rr = 0;
   DGC_CALL(dot2(s->ip_u, b, b, &bb));
-  DGC_CALL(shifted_apply(s, sigma, x, s->q));
+  DGC_CALL(dgc_ipdg_apply_spd(s->ip_u, x, s->q));
   hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 1.0, b, -1.0, (const double*)s->q, s->r, n);
+  DGC_HIP_TRY(hipGetLastError());
   DGC_CALL(dot2(s->ip_u, s->r, s->r, &rr));
-  int it = 0;
+  if (!std::isfinite(rr)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_stokes: Helmholtz residual is not finite");
   if (bb == 0.0) bb = 1.0;
+  int it = 0;
   if (rr > rtol * rtol * bb) {
-    hipLaunchKernelGGL(block_apply_kernel, grid_of(size_t(K)), dim3(256), 0, st, K, Np, (const double*)s->Binv,
-                       (const double*)s->r, s->z);
-    DGC_HIP_TRY(hipMemcpyAsync(s->d, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    DGC_CALL(dot2(s->ip_u, s->r, s->z, &rz));
-    for (it = 1; it <= maxit; ++it) {
-      DGC_CALL(shifted_apply(s, sigma, s->d, s->q));
-      DGC_CALL(dot2(s->ip_u, s->d, s->q, &pq));
-      const double alpha = rz / pq;
-      hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, alpha, (const double*)s->d, 1.0,
-                         (const double*)x, x, n);
-      hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, -alpha, (const double*)s->q, 1.0,
-                         (const double*)s->r, s->r, n);
-      DGC_CALL(dot2(s->ip_u, s->r, s->r, &rr));
-      if (!std::isfinite(rr)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_stokes: Helmholtz PCG diverged");
-      if (rr <= rtol * rtol * bb) break;
-      hipLaunchKernelGGL(block_apply_kernel, grid_of(size_t(K)), dim3(256), 0, st, K, Np, (const double*)s->Binv,
-                         (const double*)s->r, s->z);
-      double rz_new = 0;
-      DGC_CALL(dot2(s->ip_u, s->r, s->z, &rz_new));
-      hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 1.0, (const double*)s->z, rz_new / rz,
-                         (const double*)s->d, s->d, n);
-      rz = rz_new;
-    }
-    if (it > maxit) return dgc_set_error(DGC_ERR_NOT_CONVERGED, "dgc_stokes: Helmholtz PCG hit maxit");
+    double rel = 0;
+    float ms = 0;
+    DGC_CALL(dgc_ipdg_pcg(s->ip_u, s->r, s->d, rtol * std::sqrt(bb / rr), maxit, 1, &it, &rel, nullptr, &ms));
+    hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 1.0, (const double*)s->d, 1.0, (const double*)x,
+                       x, n);
+    DGC_HIP_TRY(hipGetLastError());
   }
   if (iters) *iters = it;
   return DGC_OK;
@@ -331,17 +254,15 @@ int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out) {
   if ((rc = dgc_graddiv_create(disc, Dr.data(), Ds.data(), L.data(), &s->gd))) return fail(rc);
   if ((rc = dgc_ref_operators(N, 1, nullptr, nullptr, nullptr, Dr.data(), Ds.data(), L.data(), M.data())))
     return fail(rc);
-  s->mass = M;
   if ((rc = dgc_ipdg_create(disc, Dr.data(), Ds.data(), L.data(), M.data(), 0, 1.0, &s->ip_p))) return fail(rc);
   if ((rc = dgc_ipdg_set_boundary(s->ip_p, DGC_BC_NEUMANN))) return fail(rc);
   if ((rc = dgc_ipdg_create(disc, Dr.data(), Ds.data(), L.data(), M.data(), 0, 1.0, &s->ip_u))) return fail(rc);
   const size_t n = size_t(disc->K) * Np, nb = size_t(disc->num_boundary_nodes > 0 ? disc->num_boundary_nodes : 1);
-  for (double** v : {&s->ut_x, &s->ut_y, &s->w, &s->b, &s->p, &s->gx, &s->gy, &s->r, &s->z, &s->d, &s->q, &s->tmp,
+  for (double** v : {&s->ut_x, &s->ut_y, &s->w, &s->b, &s->p, &s->gx, &s->gy, &s->r, &s->d, &s->q, &s->tmp,
                      &s->ones})
     if ((rc = salloc(s, v, n))) return fail(rc);
   for (double** v : {&s->gtx, &s->gty, &s->gux, &s->guy, &s->hq})
     if ((rc = salloc(s, v, nb))) return fail(rc);
-  if ((rc = salloc(s, &s->Binv, size_t(Np) * Np * disc->K))) return fail(rc);
   s->bx = disc->bx;
   s->by = disc->by;
   std::vector<double> one(n, 1.0);

@@ -178,6 +178,7 @@ dgc::IpdgParams<T> IpdgLevel<T>::params(const dgc_ipdg_s& s) const {
   pr.tau = tau_d;
   pr.nbr = s.disc->nbr;
   pr.neumann = s.neumann;
+  pr.shift = T(s.shift);
   pr.qx = qx;
   pr.qy = qy;
   pr.du = du;
@@ -403,6 +404,22 @@ int cg_run_fp64(dgc_ipdg_s* s, double* history, int maxit, int* iters) {
 
 static void release_coarse(dgc_ipdg_s* s);
 
+// Everything derived from the SPD operator is stale after a change of its
+// boundary kind or shift: block inverses, coarse space, captured solver
+// graphs (they hold the old parameters).
+static void operator_changed(dgc_ipdg_s* s) {
+  release_coarse(s);
+  (void)hipFree(s->f64.P);
+  s->f64.P = nullptr;
+  s->f64.use_P = false;
+  (void)hipFree(s->f32.P);
+  s->f32.P = nullptr;
+  s->f32.use_P = false;
+  if (s->f32.graph) (void)hipGraphExecDestroy(s->f32.graph);
+  s->f32.graph = nullptr;
+  s->diag_blocks.clear();
+}
+
 static void face_penalty(const dgc_disc_s* d, int tau_mode, double tau_scale, std::vector<double>& tau) {
   const int K = d->K, N = d->order;
   tau.assign(3 * size_t(K), tau_scale);
@@ -518,20 +535,17 @@ int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind) {
   if (kind == s->neumann) return DGC_OK;
   DGC_HIP_TRY(hipStreamSynchronize(s->disc->ctx->stream));
   s->neumann = kind;
-  // everything derived from the operator is stale: block inverses, coarse
-  // space, captured solver graphs (they hold the old parameters)
-  release_coarse(s);
-  for (IpdgLevel<double>* L : {&s->f64}) {
-    (void)hipFree(L->P);
-    L->P = nullptr;
-    L->use_P = false;
-  }
-  (void)hipFree(s->f32.P);
-  s->f32.P = nullptr;
-  s->f32.use_P = false;
-  if (s->f32.graph) (void)hipGraphExecDestroy(s->f32.graph);
-  s->f32.graph = nullptr;
-  s->diag_blocks.clear();
+  operator_changed(s);
+  return DGC_OK;
+}
+
+int dgc_ipdg_set_shift(dgc_ipdg_t s, double sigma) {
+  DGC_REQUIRE(s != nullptr, "dgc_ipdg_set_shift: null solver");
+  DGC_REQUIRE(std::isfinite(sigma) && sigma >= 0.0, "dgc_ipdg_set_shift: need a finite sigma >= 0");
+  if (sigma == s->shift) return DGC_OK;
+  DGC_HIP_TRY(hipStreamSynchronize(s->disc->ctx->stream));
+  s->shift = sigma;
+  operator_changed(s);
   return DGC_OK;
 }
 
@@ -802,8 +816,8 @@ int dgc_ipdg_coarse_space(dgc_ipdg_t s, const double* cell_x, const double* cell
     s->coarse_A.resize(size_t(nc) * nc);
     e = hipMemcpyAsync(s->coarse_A.data(), A, s->coarse_A.size() * sizeof(double), hipMemcpyDeviceToHost, st);
   }
-  if (e == hipSuccess && s->neumann) {
-    // With Neumann faces everywhere the constants are in the null space of S,
+  if (e == hipSuccess && s->neumann && s->shift == 0.0) {
+    // With Neumann faces everywhere (and no shift) the constants are in the null space of S,
     // and so is their coarse representation z (Phi z = 1, exact: every
     // aggregate carries the constants). Solve for z aggregate by aggregate
     // from the Gram blocks Phi_a^T Phi_a z_a = Phi_a^T 1, then invert

@@ -46,6 +46,7 @@ struct IpdgLevel {
 struct dgc_ipdg_s {
   dgc_disc_s* disc = nullptr;
   int neumann = 0;  // boundary kind of every boundary face: 0 Dirichlet, 1 Neumann
+  double shift = 0.0;  // SPD form: -M Lap_h + shift M
   int tau_mode = 0;
   double tau_scale = 1.0;
   int grid = 0;

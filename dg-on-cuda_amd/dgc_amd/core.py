@@ -94,6 +94,7 @@ def _declare(L):
     d("dgc_ipdg_set_boundary", [_vp, C.c_int])
     d("dgc_ipdg_apply_inhomogeneous", [_vp, _vp, _vp, _vp])
     d("dgc_ipdg_apply_data", [_vp, _vp, _vp, _vp])
+    d("dgc_ipdg_set_shift", [_vp, C.c_double])
     d("dgc_ipdg_assemble", [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _ip, _dp])
     d("dgc_ipdg_destroy", [_vp])
     d("dgc_ipdg_apply", [_vp, _vp])

@@ -61,6 +61,10 @@ class IPDG:
         check(lib().dgc_ipdg_apply_data(self._h, _vp(gd.ptr) if gd is not None else None,
                                         _vp(hd.ptr) if hd is not None else None, _vp(u.ptr)))
 
+    def set_shift(self, sigma: float):
+        """SPD form -M Lap_h + sigma M for apply_spd, the block-Jacobi blocks and pcg."""
+        check(lib().dgc_ipdg_set_shift(self._h, float(sigma)))
+
     def assemble(self):
         """The operator of apply() as (row_ptr, col_idx, values), CSR over the
         node-major DOF order (scipy.sparse.csr_matrix((values, col_idx, row_ptr)))."""

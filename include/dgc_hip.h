@@ -321,6 +321,13 @@ int dgc_ipdg_mass(dgc_ipdg_t s, const double* f, double* out);
  * matrix plus a rank-one term along the coarse constants. */
 enum { DGC_BC_DIRICHLET = 0, DGC_BC_NEUMANN = 1 };
 int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind);
+/* Shift of the SPD form: dgc_ipdg_apply_spd, the block-Jacobi blocks and the
+ * PCG solvers then use -M Lap_h + sigma M (sigma >= 0; 0 restores the plain
+ * form), the Helmholtz operator of the Stokes viscous step, which the
+ * reference assembles as Lap_h - (1.5/dt) I by unit vectors
+ * (examples/stokes_2d/stokes_2d.h:373-380,443-460). dgc_ipdg_apply is not
+ * shifted. A change drops the block inverses and the coarse space. */
+int dgc_ipdg_set_shift(dgc_ipdg_t s, double sigma);
 /* inout += the inhomogeneous boundary part of the Laplacian
  * (laplace_2d::apply_inhomogeneous_as_rhs, src/laplace_2d.h:253-330) for
  * boundary data per boundary node (device arrays in dgc_disc2d_arrays' bx

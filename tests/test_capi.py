@@ -71,6 +71,7 @@ def test_null_and_range_checks_without_gpu():
     assert L.dgc_ipdg_apply_inhomogeneous(None, None, None, None) == 1
     assert L.dgc_ipdg_apply_data(None, None, None, None) == 1
     assert L.dgc_ipdg_assemble(None, None, None, None, None) == 1
+    assert L.dgc_ipdg_set_shift(None, 1.0) == 1
     assert L.dgc_stokes_create(None, None) == 1
     assert L.dgc_stokes_coarse_space(None, None, None, 2, 2) == 1
     assert L.dgc_stokes_advance(None, None, None, 0.0, None, None, 0.0, 1e-3, 1e-10, None, None, None) == 1

@@ -243,3 +243,41 @@ def test_poisson_mixed_precision_refinement(ctx):
           f"FP64 PCG {r64['iters']} it {r64['ms']:.1f} ms; oracle relres {rr:.2e}")
     assert res["relres"] <= 1e-10 and rr <= 2e-10
     assert rel(xs, x64.download()) < 1e-8
+
+
+@pytest.mark.parametrize("N", [2, 3, 5, 7])
+def test_shifted_spd_form_vs_oracle(ctx, N):
+    """dgc_ipdg_set_shift: the SPD form -J M Lap_h + sigma J M against the
+    oracle's Laplacian and mass product; sigma = 0 restores the plain form bit
+    for bit; the fused PCG (first chunk captured with the shift) solves the
+    shifted system, with Dirichlet and with Neumann faces (nonsingular)."""
+    mesh = dgc_amd.Mesh.structured(9, 8, -1, 1, -1, 1, jitter=0.1, seed=N)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    K, Np = disc.K, disc.Np
+    P = Problem.from_mesh(mesh, N, form="strong")
+    M = dgc_amd.ref_operators(N, "strong")["M"]
+    J = disc.geometry["J"]
+    v = np.random.default_rng(50 + N).standard_normal(K * Np)
+    u = dgc_amd.DeviceArray.from_host(ctx, c2n(v, K, Np))
+    out = dgc_amd.DeviceArray(ctx, K * Np)
+    op.apply_spd(u, out)
+    plain = out.download()
+    sigma = 750.0
+    for kind in (0, 1):
+        op.set_boundary(kind)
+        op.set_shift(sigma)
+        op.apply_spd(u, out)
+        lap = P.laplace_bc(v, kind, 1).reshape(K, Np)
+        want = (J[:, None] * ((sigma * v.reshape(K, Np) - lap) @ M.T)).reshape(-1)
+        assert rel(n2c(out.download(), K, Np), want) < 1e-12
+        b = out.download()
+        x = dgc_amd.DeviceArray(ctx, K * Np)
+        x.zero()
+        res = op.pcg(dgc_amd.DeviceArray.from_host(ctx, b), x, rtol=1e-11, maxit=2000, precond=1)
+        assert res["relres"] <= 1e-11
+        assert rel(x.download(), c2n(v, K, Np)) < 1e-6
+        op.set_shift(0.0)
+    op.set_boundary(0)
+    op.apply_spd(u, out)
+    assert np.array_equal(out.download(), plain)
