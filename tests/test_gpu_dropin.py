@@ -93,3 +93,14 @@ def test_advection_1d_driver(tmp_path):
     assert m and float(m.group(1)) < 1e-12
     lines = plot.read_text().splitlines()
     assert lines[0] == "#         x         y" and lines[512 * 5 + 2] == "#         x         reference solution"
+
+
+@pytest.mark.parametrize("order,want", [(1, 3.3403e-05), (2, 5.4973e-08), (3, 3.6416e-11)])
+def test_poisson_driver_reproduces_reference_error_on_fine_mesh(order, want):
+    # The reference's Poisson example on square_domain_fine.msh (tau = 1): mean-square
+    # error 3.3e-5, 5.5e-8 and 3.6e-11 at N = 1, 2, 3 from its direct solve (the
+    # oracle's direct solve gives the 5 digits used here, tests/test_oracle.py);
+    # the driver's PCG solve to 1e-10 reaches the same discrete solution.
+    out = run("poisson_2d", order, os.path.join(GOLDEN, "square_domain_fine.msh"), 0)
+    m = re.search(r"error norm = ([0-9.e+-]+)", out)
+    assert m and abs(float(m.group(1)) - want) <= 1e-3 * want

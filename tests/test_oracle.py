@@ -1,6 +1,7 @@
 """Pins the oracle (the CPU restatement of the reference's serial path) to
 the golden vectors written by the compiled reference, and to properties the
 mathematics fixes where the reference has no counterpart."""
+import os
 import numpy as np
 import pytest
 
@@ -319,3 +320,40 @@ def test_stokes_oracle_converges_to_the_exact_solution():
         ex = so.exact(x, y, t)
         errs.append((np.sum((u[0] - ex[0]) ** 2) + np.sum((u[1] - ex[1]) ** 2)) / len(x))
     assert errs[0] < 5e-3 and errs[1] < errs[0] / 10
+
+
+# The reference's Poisson example (examples/poisson_2d/main.cpp:58-236: -2 pi^2
+# sin(pi x) sin(pi y) as right-hand side of the collocated Laplacian with tau = 1,
+# unit-vector assembly and a sparse direct solve; error norm = mean-square nodal
+# error against sin(pi x) sin(pi y)) run on square_domain_fine.msh (K = 1064) by
+# the compiled reference: error norm 3.3e-5 (N = 1) and 5.5e-8 (N = 2), max error
+# 1.8e-2 and 1.4e-3, 23 and 40 nonzeros per row.
+POISSON_FINE = {1: (3.3e-5, 1.8e-2, 23), 2: (5.5e-8, 1.4e-3, 40)}
+
+
+@pytest.mark.parametrize("N", [1, 2])
+def test_oracle_poisson_example_on_fine_mesh(N):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    mesh = dgc_amd.Mesh.read_gmsh(os.path.join(os.path.dirname(__file__), "golden", "square_domain_fine.msh"))
+    P = Problem.from_mesh(mesh, N, form="strong")
+    n = P.K * P.Np
+    rows, cols, vals = [], [], []
+    e = np.zeros(n)
+    for i in range(n):
+        e[i] = 1.0
+        c = P.laplace(e, 0)
+        e[i] = 0.0
+        nz = np.nonzero(c)[0]
+        rows.append(nz)
+        cols.append(np.full(len(nz), i))
+        vals.append(c[nz])
+    A = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    x, y = mesh.dof_coords(N, node_major=False)
+    exact = np.sin(np.pi * x) * np.sin(np.pi * y)
+    u = spl.spsolve(A, -2 * np.pi**2 * exact)
+    mse, mx, nnz = POISSON_FINE[N]
+    assert abs(np.sum((u - exact) ** 2) / n - mse) <= 0.05 * mse
+    assert abs(np.abs(u - exact).max() - mx) <= 0.05 * mx
+    assert round(A.nnz / n) == nnz
