@@ -104,3 +104,17 @@ def test_poisson_driver_reproduces_reference_error_on_fine_mesh(order, want):
     out = run("poisson_2d", order, os.path.join(GOLDEN, "square_domain_fine.msh"), 0)
     m = re.search(r"error norm = ([0-9.e+-]+)", out)
     assert m and abs(float(m.group(1)) - want) <= 1e-3 * want
+
+
+@pytest.mark.parametrize("order,boxes", [(1, 0), (2, 8)])
+def test_stokes_driver_on_fine_mesh(order, boxes):
+    # the Stokes example's run (examples/stokes_2d/main.cpp:58-196) through
+    # dgc_stokes_advance: 100 steps to T = 0.1 on the reference's fine mesh; the
+    # error norms of the Python path (tests/test_gpu_stokes.py): 2.535e-5 at
+    # N = 1, 4.48e-8 at N = 2 (the latter with two-level pressure solves)
+    out = run("stokes_2d", order, os.path.join(GOLDEN, "square_domain_fine.msh"), boxes)
+    m = re.search(r"T = ([0-9.e+-]+), error norm = ([0-9.e+-]+)", out)
+    assert m and abs(float(m.group(1)) - 0.1) < 1e-12
+    want = {1: 2.535e-5, 2: 4.477e-8}[order]
+    assert abs(float(m.group(2)) - want) <= 0.01 * want
+    assert "100 steps" in out
