@@ -118,3 +118,34 @@ def test_stokes_driver_on_fine_mesh(order, boxes):
     want = {1: 2.535e-5, 2: 4.477e-8}[order]
     assert abs(float(m.group(2)) - want) <= 0.01 * want
     assert "100 steps" in out
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+def test_advection_2d_driver_matches_oracle_run(order, tmp_path):
+    # the 2D advection example's run (examples/advection_2d/main.cpp:58-190): the
+    # reference's coarse mesh (K = 16), 10000 RK4 steps of its dt; the error norm
+    # equals the oracle's for the same run (the reference's CPU algorithm)
+    import numpy as np
+
+    import dgc_amd
+    from oracle_lib import Problem
+
+    path = os.path.join(GOLDEN, "square_domain_coarse.msh")
+    plot = tmp_path / "Advection2DDataFile.txt"
+    out = run("advection_2d", order, path, 10000, plot)
+    m = re.search(r"T = ([0-9.e+-]+), error norm = ([0-9.e+-]+)", out)
+    mesh = dgc_amd.Mesh.read_gmsh(path)
+    P = Problem.from_mesh(mesh, order)
+    x, y = mesh.dof_coords(order, node_major=False)
+    l = 2 / np.sqrt(mesh.K)
+    dt = (2 / 3) * (l / order) * (l / (2 + np.sqrt(2)))
+    u, _ = P.rk4(1, np.sin((x + 1) * np.pi) * np.sin((y + 1) * np.pi), 0.0, dt, nsteps=10000)
+    t = 0.0
+    for _ in range(10000):
+        t += dt
+    ref = np.sin((x + 1 - t) * np.pi) * np.sin((y + 1 - t) * np.pi)
+    want = np.sum((u - ref) ** 2) / len(x)
+    assert m and abs(float(m.group(2)) - want) <= 1e-6 * want
+    lines = plot.read_text().splitlines()
+    n = mesh.K * (order + 1) * (order + 2) // 2
+    assert lines[0] == "#         x         y         u" and len(lines) == 2 * n + 3
