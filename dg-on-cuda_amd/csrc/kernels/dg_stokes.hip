@@ -344,6 +344,8 @@ int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_p
   const double* ut[2] = {s->ut_x, s->ut_y};
   const double* gu[2] = {s->gux, s->guy};
   double* uo[2] = {ux_out, uy_out};
+  const double* un[2] = {ux, uy};
+  const double* up[2] = {ux_prev, uy_prev};
   int* its[2] = {&it_x, &it_y};
   for (int comp = 0; comp < 2; ++comp) {
     // w = -sigma u~ - A(g): the Dirichlet data's whole affine part
@@ -354,8 +356,10 @@ int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_p
     DGC_CALL(dgc_ipdg_mass(s->ip_u, s->w, s->b));
     hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, -1.0, (const double*)s->b, 0.0,
                        (const double*)s->b, s->b, n);
-    // initial guess: the projected velocity
-    DGC_HIP_TRY(hipMemcpyAsync(uo[comp], ut[comp], n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // initial guess: the linear extrapolation 2 u^n - u^{n-1} (closer to u^{n+1}
+    // than the projected velocity by a factor of order dt)
+    hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 2.0, un[comp], -1.0, up[comp], uo[comp], n);
+    DGC_HIP_TRY(hipGetLastError());
     DGC_CALL(shifted_pcg(s, sigma, s->b, uo[comp], rtol, maxit, its[comp]));
   }
   if (iters) {
