@@ -205,33 +205,39 @@ int salloc(dgc_stokes_s* s, T** p, size_t n) {
   return DGC_OK;
 }
 
-// (-M Lap_h + sigma M) x = b from the initial guess in x, to
-// |b - S x| <= rtol |b|: the shifted IPDG operator's block-Jacobi PCG (fused,
-// graph-captured iterations) solves for the correction e of r0 = b - S x0 to
-// |r0 - S e| <= (rtol |b| / |r0|) |r0|.
-int shifted_pcg(dgc_stokes_s* s, double sigma, const double* b, double* x, double rtol, int maxit, int* iters) {
+// S x = b (S the SPD form of ip) from the initial guess in x, to
+// |b - S x| <= rtol |b|: the IPDG solver's PCG (fused, graph-captured
+// iterations; precond 1 block-Jacobi, 2 two-level) solves for the correction e
+// of r0 = b - S x0 to |r0 - S e| <= (rtol |b| / |r0|) |r0|.
+int pcg_from_guess(dgc_stokes_s* s, dgc_ipdg_t ip, int precond, const double* b, double* x, double rtol, int maxit,
+                   int* iters) {
   hipStream_t st = s->disc->ctx->stream;
   const size_t n = n_of(s);
-  DGC_CALL(dgc_ipdg_set_shift(s->ip_u, sigma));
   double bb = 0, rr = 0;
-  DGC_CALL(dot2(s->ip_u, b, b, &bb));
-  DGC_CALL(dgc_ipdg_apply_spd(s->ip_u, x, s->q));
+  DGC_CALL(dot2(ip, b, b, &bb));
+  DGC_CALL(dgc_ipdg_apply_spd(ip, x, s->q));
   hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 1.0, b, -1.0, (const double*)s->q, s->r, n);
   DGC_HIP_TRY(hipGetLastError());
-  DGC_CALL(dot2(s->ip_u, s->r, s->r, &rr));
-  if (!std::isfinite(rr)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_stokes: Helmholtz residual is not finite");
+  DGC_CALL(dot2(ip, s->r, s->r, &rr));
+  if (!std::isfinite(rr)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_stokes: residual is not finite");
   if (bb == 0.0) bb = 1.0;
   int it = 0;
   if (rr > rtol * rtol * bb) {
     double rel = 0;
     float ms = 0;
-    DGC_CALL(dgc_ipdg_pcg(s->ip_u, s->r, s->d, rtol * std::sqrt(bb / rr), maxit, 1, &it, &rel, nullptr, &ms));
+    DGC_CALL(dgc_ipdg_pcg(ip, s->r, s->d, rtol * std::sqrt(bb / rr), maxit, precond, &it, &rel, nullptr, &ms));
     hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, 1.0, (const double*)s->d, 1.0, (const double*)x,
                        x, n);
     DGC_HIP_TRY(hipGetLastError());
   }
   if (iters) *iters = it;
   return DGC_OK;
+}
+
+// (-M Lap_h + sigma M) x = b (Dirichlet faces), block-Jacobi, from the guess in x
+int shifted_pcg(dgc_stokes_s* s, double sigma, const double* b, double* x, double rtol, int maxit, int* iters) {
+  DGC_CALL(dgc_ipdg_set_shift(s->ip_u, sigma));
+  return pcg_from_guess(s, s->ip_u, 1, b, x, rtol, maxit, iters);
 }
 
 }  // namespace
@@ -266,7 +272,8 @@ int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out) {
   s->bx = disc->bx;
   s->by = disc->by;
   std::vector<double> one(n, 1.0);
-  if (hipMemcpy(s->ones, one.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemset(s->p, 0, n * sizeof(double)) != hipSuccess ||
+      hipMemcpy(s->ones, one.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return fail(dgc_set_error(DGC_ERR_HIP, "dgc_stokes_create: copy failed"));
   *out = s;
   return DGC_OK;
@@ -326,12 +333,8 @@ int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_p
   DGC_CALL(dot2(s->ip_p, s->b, s->ones, &sum));
   hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, -1.0, (const double*)s->b, sum / double(n),
                      (const double*)s->ones, s->b, n);
-  DGC_HIP_TRY(hipMemsetAsync(s->p, 0, n * sizeof(double), st));
-  {
-    double rel = 0;
-    float ms = 0;
-    DGC_CALL(dgc_ipdg_pcg(s->ip_p, s->b, s->p, rtol, maxit, s->pressure_precond, &it_p, &rel, nullptr, &ms));
-  }
+  //    from the previous step's pressure (zero before the first step)
+  DGC_CALL(pcg_from_guess(s, s->ip_p, s->pressure_precond, s->b, s->p, rtol, maxit, &it_p));
   // 3. projection: u~ -= (2 dt / 3) grad p, pressure BC exterior = interior (stokes_2d.h:340-349)
   DGC_CALL(dgc_grad2d_apply(s->gd, s->p, nullptr, 1.0, s->gx, s->gy));
   hipLaunchKernelGGL(axpby_kernel, grid_of(n), dim3(256), 0, st, -2. * dt / 3., (const double*)s->gx, 1.0,
