@@ -936,12 +936,12 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
     if (L.graph) (void)hipGraphExecDestroy(L.graph);
     L.graph = nullptr;
   }
-  // ||b||^2 on the host (set-up, outside the timed region)
-  std::vector<double> hb(n);
-  DGC_HIP_TRY(hipMemcpyAsync(hb.data(), b, n * sizeof(double), hipMemcpyDeviceToHost, st));
-  DGC_HIP_TRY(hipStreamSynchronize(st));
+  // ||b||^2 on the device (set-up, outside the timed region)
   double bb = 0.0;
-  for (double v : hb) bb += v * v;
+  {
+    int rc = dot2(s, b, b, &bb);
+    if (rc) return rc;
+  }
   DGC_REQUIRE(std::isfinite(bb), "dgc_ipdg_pcg: non-finite right-hand side");
   if (bb == 0.0) {
     DGC_HIP_TRY(hipMemsetAsync(x, 0, n * sizeof(double), st));
