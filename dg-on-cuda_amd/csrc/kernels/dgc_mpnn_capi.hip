@@ -577,11 +577,12 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
     m->graph = nullptr;
   }
-  std::vector<double> hb(n);
-  DGC_HIP_TRY(hipMemcpyAsync(hb.data(), b, n * sizeof(double), hipMemcpyDeviceToHost, st));
-  DGC_HIP_TRY(hipStreamSynchronize(st));
+  // ||b||^2 on the device, as dgc_ipdg_pcg forms it
   double bb = 0.0;
-  for (double v : hb) bb += v * v;
+  {
+    int rc = dot2(s, b, b, &bb);
+    if (rc) return rc;
+  }
   DGC_REQUIRE(std::isfinite(bb) && bb > 0, "dgc_ipdg_fcg: zero or non-finite right-hand side");
   const double nd = double(n);
 
