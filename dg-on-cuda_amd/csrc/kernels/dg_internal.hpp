@@ -34,6 +34,20 @@ struct dgc_disc_s {
 
 namespace dgc {
 
+// Two HIP events (start, stop) destroyed on every exit path of the timed
+// entry points.
+struct EventPair {
+  hipEvent_t start = nullptr, stop = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&start);
+    return e == hipSuccess ? hipEventCreate(&stop) : e;
+  }
+  ~EventPair() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+};
+
 enum Physics { kMaxwellTM = 0, kAdvection = 1, kPhysicsCount = 2 };
 enum AdvectionBC { kBcSinSin = 0, kBcZero = 1 };
 

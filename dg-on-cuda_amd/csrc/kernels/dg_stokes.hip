@@ -287,6 +287,12 @@ int dgc_stokes_coarse_space(dgc_stokes_t s, const double* cell_x, const double* 
   return DGC_OK;
 }
 
+int dgc_stokes_reset(dgc_stokes_t s) {
+  DGC_REQUIRE(s != nullptr, "dgc_stokes_reset: null solver");
+  DGC_HIP_TRY(hipMemsetAsync(s->p, 0, n_of(s) * sizeof(double), s->disc->ctx->stream));
+  return DGC_OK;
+}
+
 int dgc_stokes_destroy(dgc_stokes_t s) {
   if (s == nullptr) return DGC_OK;
   (void)hipStreamSynchronize(s->disc->ctx->stream);

@@ -49,6 +49,7 @@ struct dgc_op_s {
   std::vector<double> hDr, hDs, hL;  // host copies of the operators
   int cache_policy = DGC_CACHE_AUTO;
   double* tclock = nullptr;
+  int* flag = nullptr;  // non-finite flag of the time loops' state check
   // cached graph of one full RK4 step
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
@@ -266,6 +267,7 @@ static int upload_operators(dgc_op_s* op, const double* Dr, const double* Ds, co
   DGC_HIP_TRY(hipMalloc(&op->DsT, dsT.size() * sizeof(double)));
   DGC_HIP_TRY(hipMalloc(&op->LT, lT.size() * sizeof(double)));
   DGC_HIP_TRY(hipMalloc(&op->tclock, 2 * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&op->flag, sizeof(int)));
   DGC_HIP_TRY(hipMemcpy(op->DrT, drT.data(), drT.size() * sizeof(double), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(op->DsT, dsT.data(), dsT.size() * sizeof(double), hipMemcpyHostToDevice));
   DGC_HIP_TRY(hipMemcpy(op->LT, lT.data(), lT.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -313,6 +315,7 @@ static void free_op(dgc_op_s* op) {
   (void)hipFree(op->DsT);
   (void)hipFree(op->LT);
   (void)hipFree(op->tclock);
+  (void)hipFree(op->flag);
   delete op;
 }
 
@@ -517,6 +520,43 @@ static int check_step_args(dgc_op_s* op, double* const* inout, size_t size, doub
   return DGC_OK;
 }
 
+struct FieldPtrs {
+  const double* f[3];
+};
+
+__global__ void flag_clear_kernel(int* flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) flag[0] = 0;
+}
+
+// flag |= 1 if any of the F fields of n values holds a NaN or an infinity
+// (one atomic per wave that saw one).
+__global__ void nonfinite_scan_kernel(FieldPtrs p, int F, size_t n, int* flag) {
+  bool bad = false;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    for (int f = 0; f < F; ++f) bad |= !isfinite(p.f[f][i]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+// The time loops' state check (the reference's has_nan helper,
+// examples/advection_2d/main.cpp:53-58, applied once per call): one pass over
+// the state after the queued steps; blocks until it is known.
+static int check_state_finite(dgc_op_s* op, double* const* u, const char* what) {
+  hipStream_t s = op->disc->ctx->stream;
+  FieldPtrs fp{};
+  for (int f = 0; f < op->F; ++f) fp.f[f] = u[f];
+  const size_t n = size_t(op->disc->K) * op->disc->Np;
+  const int grid = int(std::min<size_t>((n + 255) / 256, 2048));
+  hipLaunchKernelGGL(flag_clear_kernel, dim3(1), dim3(64), 0, s, op->flag);
+  hipLaunchKernelGGL(nonfinite_scan_kernel, dim3(grid), dim3(256), 0, s, fp, op->F, n, op->flag);
+  DGC_HIP_TRY(hipGetLastError());
+  int h = 0;
+  DGC_HIP_TRY(hipMemcpyAsync(&h, op->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  DGC_HIP_TRY(hipStreamSynchronize(s));
+  if (h != 0) return dgc_set_error(DGC_ERR_NONFINITE, std::string(what) + ": the state holds NaN or Inf values");
+  return DGC_OK;
+}
+
 int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, double dt,
                  double* const* wk) {
   int rc = check_step_args(op, inout, size, wk, "dgc_rk4_step");
@@ -582,7 +622,9 @@ int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, dou
   if (rc != DGC_OK) return rc;
   DGC_REQUIRE(nsteps >= 0, "dgc_rk4_steps: negative step count");
   DGC_REQUIRE(std::isfinite(dt) && dt > 0.0, "dgc_rk4_steps: dt must be positive");
-  return run_full_steps(op, inout, t0, dt, nsteps, wk);
+  rc = run_full_steps(op, inout, t0, dt, nsteps, wk);
+  if (rc != DGC_OK) return rc;
+  return check_state_finite(op, inout, "dgc_rk4_steps");
 }
 
 int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
@@ -607,7 +649,7 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (err != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("timed replay: ") + hipGetErrorString(err));
-  return DGC_OK;
+  return check_state_finite(op, inout, "dgc_rk4_steps_timed");
 }
 
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
@@ -777,7 +819,7 @@ int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, doubl
   }
   if (t_out) *t_out = t;
   if (steps_out) *steps_out = steps;
-  return DGC_OK;
+  return check_state_finite(op, inout, "dgc_rk4_run");
 }
 
 }  // extern "C"

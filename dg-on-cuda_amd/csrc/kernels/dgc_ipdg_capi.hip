@@ -649,6 +649,13 @@ int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors) {
       for (int k = j; k < Np; ++k)
         hinv[size_t(dgc::packed_index(Np, j, k)) * K + c] = 0.5 * (Bi(j, k) + Bi(k, j));
   }
+  // captured solver graphs (PCG at both precisions, the network's FCG) hold
+  // the old inverses' addresses
+  if (s->f64.graph) (void)hipGraphExecDestroy(s->f64.graph);
+  s->f64.graph = nullptr;
+  if (s->f32.graph) (void)hipGraphExecDestroy(s->f32.graph);
+  s->f32.graph = nullptr;
+  ++s->coarse_gen;
   (void)hipFree(s->f64.P);
   (void)hipFree(s->f32.P);
   s->f64.P = nullptr;
@@ -949,10 +956,9 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
     if (relres) *relres = 0.0;
     return DGC_OK;
   }
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  DGC_HIP_TRY(hipEventRecord(e0, st));
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  DGC_HIP_TRY(hipEventRecord(ev.start, st));
   DGC_HIP_TRY(hipMemsetAsync(L.x, 0, n * sizeof(double), st));
   int total = 0, rc = DGC_OK;
   double rel = 1.0;
@@ -973,12 +979,10 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
     }
     if (rel <= rtol || total >= maxit) break;
   }
-  DGC_HIP_TRY(hipEventRecord(e1, st));
-  DGC_HIP_TRY(hipEventSynchronize(e1));
+  DGC_HIP_TRY(hipEventRecord(ev.stop, st));
+  DGC_HIP_TRY(hipEventSynchronize(ev.stop));
   float elapsed = 0.f;
-  (void)hipEventElapsedTime(&elapsed, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&elapsed, ev.start, ev.stop);
   if (rc != DGC_OK) return rc;
   DGC_HIP_TRY(hipMemcpyAsync(x, L.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (history) DGC_HIP_TRY(hipMemcpyAsync(history, s->hist, (total + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1010,10 +1014,9 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
   int rc = dot2(s, b, b, &bb);
   if (rc) return rc;
   DGC_REQUIRE(std::isfinite(bb), "dgc_ipdg_pcg_mixed: non-finite right-hand side");
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  DGC_HIP_TRY(hipEventRecord(e0, st));
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  DGC_HIP_TRY(hipEventRecord(ev.start, st));
   DGC_HIP_TRY(hipMemsetAsync(H.x, 0, n * sizeof(double), st));
   DGC_HIP_TRY(hipMemcpyAsync(H.r, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   double rr = bb, rel = bb > 0 ? 1.0 : 0.0;
@@ -1049,12 +1052,10 @@ int dgc_ipdg_pcg_mixed(dgc_ipdg_t s, const double* b, double* x, double rtol, do
     ++outer;
     if (!(rel == rel)) return dgc_set_error(DGC_ERR_NONFINITE, "dgc_ipdg_pcg_mixed: residual is not finite");
   }
-  DGC_HIP_TRY(hipEventRecord(e1, st));
-  DGC_HIP_TRY(hipEventSynchronize(e1));
+  DGC_HIP_TRY(hipEventRecord(ev.stop, st));
+  DGC_HIP_TRY(hipEventSynchronize(ev.stop));
   float elapsed = 0.f;
-  (void)hipEventElapsedTime(&elapsed, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&elapsed, ev.start, ev.stop);
   DGC_HIP_TRY(hipMemcpyAsync(x, H.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   DGC_HIP_TRY(hipStreamSynchronize(st));
   if (outer_iters) *outer_iters = outer;

@@ -62,7 +62,7 @@ struct dgc_ipdg_s {
   dgc::CoarseSpace coarse;
   std::vector<void*> coarse_owned;
   std::vector<double> coarse_A;  // Phi^T S Phi (host copy, row-major)
-  int coarse_gen = 0;            // bumped whenever the coarse arrays change
+  int coarse_gen = 0;            // bumped whenever arrays that solver graphs capture change
 };
 
 // z += Phi A_c^{-1} Phi^T r with partials (r.z, r.r) or (z.z, r.r) into `partial`.

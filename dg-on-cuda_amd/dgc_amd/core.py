@@ -72,6 +72,7 @@ def _declare(L):
     d("dgc_adv1d_rk4_steps", [_vp, _vp, C.c_double, C.c_double, C.c_int])
     d("dgc_write_plot_file", [C.c_char_p, C.c_size_t, _dp, _dp, _dp, _dp])
     d("dgc_stokes_destroy", [_vp])
+    d("dgc_stokes_reset", [_vp])
     d("dgc_stokes_coarse_space", [_vp, _dp, _dp, C.c_int, C.c_int])
     d("dgc_stokes_advance", [_vp, _vp, _vp, C.c_double, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp,
                              _ip])

@@ -32,6 +32,10 @@ class Stokes:
         c = mesh.cells()
         check(lib().dgc_stokes_coarse_space(self._h, _d(c["cx"]), _d(c["cy"]), gx, gy if gy else gx))
 
+    def reset(self):
+        """Forget the retained pressure (a new trajectory starts from p = 0)."""
+        check(lib().dgc_stokes_reset(self._h))
+
     def advance(self, u_prev, t_prev, u, t, dt, out, rtol=1e-12) -> dict:
         """u_prev, u, out: pairs of DeviceArray (x, y components), node-major."""
         it = np.zeros(3, np.int32)

@@ -165,11 +165,18 @@ int dgc_div2d_apply(dgc_graddiv_t op, const double* u0, const double* u1, const 
  * GPU, the path the reference leaves as a TODO (k_stokes_2d.cu:96-106): from
  * (u^{n-1}, t_prev) and (u^n, t) to u^{n+1} at t + dt for the example's
  * analytic problem (a = 2.883356; its boundary data, stokes_2d.h:100-183),
- * with tau = 1 as the reference. Pressure: block-Jacobi PCG on -M Lap_h with
- * Neumann faces, right-hand side projected onto the complement of the
- * constants (the reference factorises the singular matrix directly);
- * viscous step: block-Jacobi PCG on -M Lap_h + (3 / 2dt) M with Dirichlet
- * faces, one solve per component. rtol: relative residual of every solve.
+ * with tau = 1 as the reference. Pressure: PCG on -M Lap_h with Neumann
+ * faces (block-Jacobi, or two-level after dgc_stokes_coarse_space),
+ * right-hand side projected onto the complement of the constants (the
+ * reference factorises the singular matrix directly); viscous step:
+ * block-Jacobi PCG on -M Lap_h + (3 / 2dt) M with Dirichlet faces, one solve
+ * per component. rtol: relative residual of every solve.
+ * Retained state: the handle keeps the last pressure, and each pressure
+ * solve starts from it (zero before the first call); a new trajectory starts
+ * from a fresh handle or after dgc_stokes_reset. The viscous solves start
+ * from 2 u^n - u^{n-1}, written into ux_out / uy_out before the solves:
+ * ux_out may alias ux or ux_prev (and uy_out uy or uy_prev), but an output
+ * must not alias an input of the other component.
  * Velocities: device arrays, node-major [Np][K]. iters (may be NULL): the
  * pressure, x and y iteration counts. Synchronous. */
 int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out);
@@ -177,6 +184,8 @@ int dgc_stokes_create(dgc_disc_t disc, dgc_stokes_t* out);
  * pressure operator (cell corners cx, cy [3K] of the mesh, gx x gy boxes). */
 int dgc_stokes_coarse_space(dgc_stokes_t s, const double* cell_x, const double* cell_y, int gx, int gy);
 int dgc_stokes_destroy(dgc_stokes_t s);
+/* Forgets the retained pressure (the next pressure solve starts from zero). */
+int dgc_stokes_reset(dgc_stokes_t s);
 int dgc_stokes_advance(dgc_stokes_t s, const double* ux_prev, const double* uy_prev, double t_prev,
                        const double* ux, const double* uy, double t, double dt, double rtol,
                        double* ux_out, double* uy_out, int* iters);
@@ -276,11 +285,14 @@ int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, doubl
 /* Time loop t0 -> t_end with step dt, the last step truncated to land on
  * t_end (examples/maxwell_2d/main.cpp:199-209). Full steps are replayed from a
  * captured hipGraph. *t_out / *steps_out report the final time and the step
- * count (either may be NULL). */
+ * count (either may be NULL). Blocks until the steps are done, then scans the
+ * state once: DGC_ERR_NONFINITE if it holds a NaN or an infinity (the
+ * reference's has_nan helper, examples/advection_2d/main.cpp:53-58). */
 int dgc_rk4_run(dgc_op_t op, double* const* inout, size_t size, double t0, double t_end, double dt,
                 double* const* wk, double* t_out, int* steps_out);
 
-/* Fixed number of full steps (examples/advection_2d/main.cpp:135-152), graph-replayed. */
+/* Fixed number of full steps (examples/advection_2d/main.cpp:135-152),
+ * graph-replayed; blocks and checks the state as dgc_rk4_run does. */
 int dgc_rk4_steps(dgc_op_t op, double* const* inout, size_t size, double t0, double dt, int nsteps,
                   double* const* wk);
 
@@ -449,8 +461,9 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
 int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C);
 
 /* Same as dgc_rk4_steps, bracketed by HIP events on the context stream;
- * blocks until done and returns the elapsed milliseconds in *ms
- * (measurement entry point; no reference counterpart). */
+ * blocks until done and returns the elapsed milliseconds in *ms (the state
+ * check runs after the timed region; measurement entry point, no reference
+ * counterpart). */
 int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t0, double dt,
                         int nsteps, double* const* wk, float* ms);
 

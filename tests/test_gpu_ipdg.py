@@ -119,6 +119,27 @@ def test_pcg_small_matches_direct_solve(ctx, precond):
     assert np.all(np.isfinite(res["history"]))
 
 
+def test_rebuilt_block_jacobi_invalidates_captured_solver_graphs(ctx):
+    """A second dgc_ipdg_block_jacobi frees and re-uploads the inverses: the
+    captured PCG graphs that held the old addresses must be rebuilt, so a
+    solve after the rebuild gives the same iterate as the one before it."""
+    N = 2
+    mesh = dgc_amd.Mesh.structured(6, 6, -1, 1, -1, 1, jitter=0.1, seed=5)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    K, Np = disc.K, disc.Np
+    b = dgc_amd.DeviceArray.from_host(ctx, np.random.default_rng(3).standard_normal(K * Np))
+    x1, x2 = dgc_amd.DeviceArray(ctx, K * Np), dgc_amd.DeviceArray(ctx, K * Np)
+    r1 = op.pcg(b, x1, rtol=1e-12, maxit=5000, precond=1)
+    op.block_jacobi()
+    spacer = dgc_amd.DeviceArray(ctx, K * Np * Np)  # moves later allocations elsewhere
+    op.block_jacobi()
+    r2 = op.pcg(b, x2, rtol=1e-12, maxit=5000, precond=1)
+    del spacer
+    assert r1["iters"] == r2["iters"]
+    np.testing.assert_array_equal(x1.download(), x2.download())
+
+
 def _textbook_pcg(S, b, precond, iters):
     """Residual norms of textbook PCG (x0 = 0) on a dense matrix."""
     x = np.zeros_like(b)

@@ -242,6 +242,84 @@ def test_periodic_mass_conservation_full_config2_size(ctx):
     assert np.sum(u**2) <= np.sum(u0**2) * 1.0001  # no growth
 
 
+@pytest.fixture(scope="module")
+def c2_mesh():
+    return dgc_amd.Mesh.structured(256, 256, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+
+
+# RMS-relative error of the oracle's own 200-step run against the exact
+# translate of the Gaussian on this mesh (measured: 3.66e-4, 5.01e-6,
+# 4.33e-8, 3.43e-10, 6.0e-12, 5.5e-12, 5.5e-12 for N = 1..7), with 50 % margin.
+C2_EXACT_BOUND = {1: 5.5e-4, 2: 7.5e-6, 3: 6.5e-8, 4: 5.2e-10, 5: 9e-12, 6: 9e-12, 7: 9e-12}
+
+
+def _gauss_translate(x, y, T, ax, ay):
+    out = np.zeros_like(x)
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            out += np.exp(-((x - ax * T - 0.5 + i) ** 2 + (y - ay * T - 0.5 + j) ** 2) / 0.01)
+    return out
+
+
+@pytest.mark.parametrize("N", range(1, 8))
+def test_config2_full_size_vs_oracle_and_exact_translation(ctx, c2_mesh, N):
+    """The advection workload at its full size (256 x 256 periodic squares,
+    K = 131072, a = (1, 0.5), Gaussian, CFL 0.5, 200 RK4 steps) with the
+    autotuned tiling: the RHS and the first 5 steps against the oracle
+    (FP64 bar), all 200 steps against the oracle for N <= 3, and the final
+    state against the exact translate of the Gaussian for every N."""
+    mesh = c2_mesh
+    print(f"C2 full size N={N}", flush=True)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO)
+    K, Np = disc.K, disc.Np
+    x, y = mesh.dof_coords(N, node_major=False)
+    u0 = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    dt = 0.5 * mesh.min_edge / (np.hypot(1.0, 0.5) * (N + 1) ** 2)
+    P = Problem.from_mesh(mesh, N, threaded=True)
+    state = upload_fields(ctx, u0, 1, K, Np)
+    work = op.new_work()
+    op.autotune(state, work, reps=2)
+    dout = [dgc_amd.DeviceArray(ctx, K * Np)]
+    op.rhs(state, 0.0, dout)
+    assert rel(download_fields(dout, K, Np), P.advection_rhs(u0, 0.0, 1.0, 0.5, 1)) < TOL
+    op.rk4_steps(state, 0.0, dt, 5, work)
+    want5, _ = P.rk4(1, u0, 0.0, dt, nsteps=5, ax=1.0, ay=0.5, bc=1)
+    assert rel(download_fields(state, K, Np), want5) < TOL
+    op.rk4_steps(state, 5 * dt, dt, 195, work)
+    got = download_fields(state, K, Np)
+    if N <= 3:
+        want, _ = P.rk4(1, u0, 0.0, dt, nsteps=200, ax=1.0, ay=0.5, bc=1)
+        assert rel(got, want) < TOL
+    err = rel(got, _gauss_translate(x, y, 200 * dt, 1.0, 0.5))
+    print(f"N={N} tiling {op.waves_per_tile}: rel L2 vs exact translate after 200 steps {err:.3e}")
+    assert err < C2_EXACT_BOUND[N]
+
+
+def test_time_loops_report_a_nonfinite_state(ctx, c1):
+    """A NaN or an infinity in the state (plain data written from the host)
+    makes dgc_rk4_steps / dgc_rk4_run return DGC_ERR_NONFINITE; a finite
+    state passes the same check."""
+    fix, mesh, disc, op = c1
+    K, Np = disc.K, disc.Np
+    dt = float(fix["dt"])
+    work = op.new_work()
+    for bad in (np.nan, np.inf):
+        u = np.random.default_rng(5).standard_normal(3 * K * Np)
+        u[2 * K * Np + 17] = bad  # one Ez value
+        state = upload_fields(ctx, u, 3, K, Np)
+        with pytest.raises(dgc_amd.DGCError) as ei:
+            op.rk4_steps(state, 0.0, dt, 3, work)
+        assert ei.value.code == 4 and "NaN" in str(ei.value)
+        state = upload_fields(ctx, u, 3, K, Np)
+        with pytest.raises(dgc_amd.DGCError) as ei:
+            op.rk4_run(state, 0.0, 3.5 * dt, dt, work)
+        assert ei.value.code == 4
+    state = upload_fields(ctx, np.random.default_rng(5).standard_normal(3 * K * Np), 3, K, Np)
+    op.rk4_steps(state, 0.0, dt, 3, work)
+    assert np.all(np.isfinite(download_fields(state, K, Np)))
+
+
 def test_bad_arguments_are_errors(ctx, c1):
     fix, mesh, disc, op = c1
     a = [dgc_amd.DeviceArray(ctx, disc.size) for _ in range(3)]

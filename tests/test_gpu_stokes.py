@@ -110,3 +110,25 @@ def test_stokes_with_two_level_pressure_solves(ctx):
     assert rel(bx, ax) < 1e-8 and rel(by, ay) < 1e-8
     assert its2[-1]["pressure"] < its1[-1]["pressure"] / 2
 
+
+
+def test_stokes_reset_forgets_the_retained_pressure(ctx):
+    """Each pressure solve starts from the handle's last pressure; after
+    dgc_stokes_reset the first step of a new trajectory repeats the fresh
+    handle's first step exactly."""
+    mesh = dgc_amd.Mesh.structured(6, 6, -1, 1, -1, 1, jitter=0.1, seed=3)
+    N, dt = 2, 1e-3
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    st = dgc_amd.Stokes(disc)
+    x, y = mesh.dof_coords(N, node_major=True)
+    ux, uy = so.exact(x, y, 0.0)
+    u = (dgc_amd.DeviceArray.from_host(ctx, ux), dgc_amd.DeviceArray.from_host(ctx, uy))
+    out = (dgc_amd.DeviceArray(ctx, disc.size), dgc_amd.DeviceArray(ctx, disc.size))
+    st.advance(u, 0.0, u, 0.0, dt, out, 1e-13)
+    first = (out[0].download(), out[1].download())
+    for _ in range(2):
+        st.advance(u, 0.0, u, 0.0, dt, out, 1e-13)
+    st.reset()
+    st.advance(u, 0.0, u, 0.0, dt, out, 1e-13)
+    np.testing.assert_array_equal(out[0].download(), first[0])
+    np.testing.assert_array_equal(out[1].download(), first[1])

@@ -357,3 +357,116 @@ def test_oracle_poisson_example_on_fine_mesh(N):
     assert abs(np.sum((u - exact) ** 2) / n - mse) <= 0.05 * mse
     assert abs(np.abs(u - exact).max() - mx) <= 0.05 * mx
     assert round(A.nnz / n) == nnz
+
+
+# ---- the generalised advection operator (a != (1, 1), periodic) -------------
+# The reference hard-codes a = (1, 1) (examples/advection_2d/advection_2d.h:216-218),
+# so the oracle's general velocity is pinned by what the mathematics fixes:
+# consistency with -a.grad u, the exact upwind energy identity, and the
+# convergence of a periodic translation to the exact translated field.
+
+def _periodic(n, N, jitter=0.05, ny=None):
+    m = dgc_amd.Mesh.structured(n, ny or n, 0, 1, 0, 1, jitter=jitter, seed=1, periodic=True)
+    return m, Problem.from_mesh(m, N)
+
+
+def _sine_field(x, y):
+    u = np.sin(2 * np.pi * x) * np.cos(4 * np.pi * y)
+    ux = 2 * np.pi * np.cos(2 * np.pi * x) * np.cos(4 * np.pi * y)
+    uy = -4 * np.pi * np.sin(2 * np.pi * x) * np.sin(4 * np.pi * y)
+    return u, ux, uy
+
+
+@pytest.mark.parametrize("a", [(1.0, 0.5), (-0.3, 0.8)])
+def test_advection_rhs_is_minus_a_dot_grad(a):
+    """On the periodic 24 x 24 mesh at N = 7 the RHS of a smooth periodic field
+    equals -(a . grad u) to the discretization error (O(h^N): 5e-9 here, and
+    1/2^N of the 12 x 12 error); a swapped (ax, ay) or a flipped ay is O(1) off."""
+    ax, ay = a
+    errs = []
+    for n in (12, 24):
+        m, P = _periodic(n, 7)
+        x, y = m.dof_coords(7, node_major=False)
+        u, ux, uy = _sine_field(x, y)
+        exact = -(ax * ux + ay * uy)
+        errs.append(rel(P.advection_rhs(u, 0.0, ax, ay, 1), exact))
+        if n == 24:
+            assert errs[-1] < 1e-7
+            assert rel(P.advection_rhs(u, 0.0, ay, ax, 1), exact) > 0.5    # planted: swapped components
+            assert rel(P.advection_rhs(u, 0.0, ax, -ay, 1), exact) > 0.5   # planted: wrong sign of ay
+    assert errs[0] / errs[1] > 0.7 * 2**7
+
+
+def _face_mass(xi):
+    """1D mass matrix at the face nodes xi of [-1, 1]: (V V^T)^{-1} with the
+    orthonormal Legendre Vandermonde."""
+    from numpy.polynomial import legendre
+
+    n = len(xi)
+    V = np.stack([legendre.legval(xi, np.eye(n)[k]) * np.sqrt((2 * k + 1) / 2) for k in range(n)], axis=1)
+    return np.linalg.inv(V @ V.T)
+
+
+def _upwind_energy_rate(N, ax, ay, u, m, ops):
+    """(d/dt of 1/2 |u|_M^2 under the oracle's RHS, the closed form
+    -1/2 sum over faces of |a.n| int [u]^2 ds). For the weak form with upwind
+    flux on a periodic mesh the two are equal identically in u."""
+    P = Problem.from_mesh(m, N, ops=ops)
+    K, Np = P.K, P.Np
+    J = m.geometry(N)["J"]
+    c = m.cells()
+    cx, cy = c["cx"].reshape(K, 3), c["cy"].reshape(K, 3)
+    nbc, nbf = c["nb_cell"].reshape(K, 3), c["nb_face"].reshape(K, 3)
+    fm, r, s = ops["fmask"], ops["r"], ops["s"]
+    M1 = [_face_mass(r[fm[0]]), _face_mass(s[fm[1]]), _face_mass(s[fm[2]])]
+    uc = u.reshape(K, Np)
+    duc = P.advection_rhs(u, 0.0, ax, ay, 1).reshape(K, Np)
+    rate = sum(J[k] * uc[k] @ ops["M"] @ duc[k] for k in range(K))
+    diss = 0.0
+    for k in range(K):
+        for e in range(3):
+            dx, dy = cx[k, (e + 1) % 3] - cx[k, e], cy[k, (e + 1) % 3] - cy[k, e]
+            ln = np.hypot(dx, dy)
+            an = (ax * dy - ay * dx) / ln  # a . outward normal of a counter-clockwise cell
+            jmp = uc[k, fm[e]] - uc[nbc[k, e], fm[nbf[k, e]][::-1]]
+            diss += abs(an) * 0.5 * ln * (jmp @ M1[e] @ jmp)
+    return rate, -0.25 * diss  # every face is visited from both sides
+
+
+@pytest.mark.parametrize("N", [1, 3, 7])
+@pytest.mark.parametrize("a", [(1.0, 0.5), (-0.3, 0.8)])
+def test_advection_upwind_energy_identity(N, a):
+    """The semi-discrete energy rate of a random field equals the upwind
+    dissipation -1/2 sum |a.n| int [u]^2 to rounding; a central or downwind
+    flux, a wrong flux scale or a wrong velocity breaks it."""
+    m = dgc_amd.Mesh.structured(6, 5, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+    ops = dgc_amd.ref_operators(N, "weak")
+    P = Problem.from_mesh(m, N, ops=ops)
+    u = np.random.default_rng(N).standard_normal(P.K * P.Np)
+    rate, closed = _upwind_energy_rate(N, a[0], a[1], u, m, ops)
+    assert abs(rate - closed) < 1e-12 * abs(closed)
+    # planted: the closed form of the swapped velocity is a different number
+    _, swapped = _upwind_energy_rate(N, a[1], a[0], u, m, ops)
+    assert abs(rate - swapped) > 1e-3 * abs(closed)
+
+
+def _translation_error(n, N, ax, ay, T=0.25, ex_a=None):
+    m, P = _periodic(n, N)
+    x, y = m.dof_coords(N, node_major=False)
+    f = lambda x, y: np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)  # noqa: E731
+    dt0 = 0.25 * m.min_edge / (np.hypot(ax, ay) * (N + 1) ** 2)
+    ns = int(np.ceil(T / dt0))
+    u, _ = P.rk4(1, f(x, y), 0.0, T / ns, nsteps=ns, ax=ax, ay=ay, bc=1)
+    bx, by = ex_a if ex_a is not None else (ax, ay)
+    return np.sqrt(np.mean((u - f(x - bx * T, y - by * T)) ** 2))
+
+
+@pytest.mark.parametrize("a", [(1.0, 0.5), (-0.3, 0.8)])
+def test_periodic_translation_converges_at_order_N_plus_1(a):
+    """RK4 of the oracle on periodic meshes translates a periodic sine by a T:
+    the RMS error against the exact translate falls by about 2^(N+1) per
+    halving of h (N = 2: 8x), while the translate by the swapped velocity
+    stays O(1) away."""
+    e = [_translation_error(n, 2, *a) for n in (8, 16, 32)]
+    assert e[0] / e[1] > 6.0 and e[1] / e[2] > 6.0 and e[2] < 3e-4
+    assert _translation_error(16, 2, *a, ex_a=(a[1], a[0])) > 0.05
