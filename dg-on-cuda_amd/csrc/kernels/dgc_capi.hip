@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,19 @@
 namespace {
 thread_local std::string g_last_error;
 }
+
+// ---- work area of dgc_rk4_step_noalloc --------------------------------------
+// A file-scope device array: every GPU that loads the library holds its own
+// copy, at its own address, until the process ends. Host code never uses the
+// array's name as a device address: dgc_ctx_create looks the address up on
+// its device (hipGetSymbolAddress) and keeps it per device in g_scratch_ptr.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
 
 int dgc_set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -68,6 +82,16 @@ int dgc_ctx_create(int device, dgc_ctx_t* out) {
   DGC_HIP_TRY(hipGetDeviceCount(&n));
   DGC_REQUIRE(device >= 0 && device < n, "dgc_ctx_create: no such device");
   DGC_HIP_TRY(hipSetDevice(device));
+  DGC_REQUIRE(device < kMaxDevices, "dgc_ctx_create: device index above 63");
+  {
+    // once per device, with that device current: the address of its work area
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device] == nullptr) {
+      void* ptr = nullptr;
+      DGC_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+  }
   auto* ctx = new dgc_ctx_s;
   ctx->device = device;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -561,6 +585,30 @@ int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, doubl
                  double* const* wk) {
   int rc = check_step_args(op, inout, size, wk, "dgc_rk4_step");
   if (rc != DGC_OK) return rc;
+  return enqueue_step(op, inout, t, dt, wk, false);
+}
+
+// dgc_rk4_step with the work vectors in the device's static work area: no
+// allocation, no synchronisation. Every stage writes each work vector it
+// reads later in the step, so nothing needs clearing between calls.
+int dgc_rk4_step_noalloc(dgc_op_t op, double* const* inout, size_t size, double t, double dt) {
+  DGC_REQUIRE(op != nullptr, "dgc_rk4_step_noalloc: null operator");
+  DGC_REQUIRE(size == size_t(op->disc->K) * op->disc->Np, "dgc_rk4_step_noalloc: size != K * Np");
+  int rc = check_fields(op, inout, "dgc_rk4_step_noalloc");
+  if (rc != DGC_OK) return rc;
+  const int F = op->F;
+  if (size_t(4) * F * size > kScratchBytes / sizeof(double))
+    return dgc_set_error(DGC_ERR_INVALID, "dgc_rk4_step_noalloc: 4 * F * K * Np = " +
+                                              std::to_string(size_t(4) * F * size) +
+                                              " doubles exceed the 64 MiB work area (F * K * Np <= 2097152)");
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[op->disc->ctx->device];
+  }
+  if (base == nullptr) return dgc_set_error(DGC_ERR_INVALID, "dgc_rk4_step_noalloc: work area not set up");
+  double* wk[12] = {};
+  for (int i = 0; i < 4 * F; ++i) wk[i] = base + size_t(i) * size;
   return enqueue_step(op, inout, t, dt, wk, false);
 }
 

@@ -282,6 +282,16 @@ int dgc_axpy(dgc_ctx_t ctx, double a, const double* x, size_t n, const double* y
 int dgc_rk4_step(dgc_op_t op, double* const* inout, size_t size, double t, double dt,
                  double* const* wk);
 
+/* dgc_rk4_step without allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). The 4*F work vectors live
+ * in a static 64 MiB device array of the library, one per GPU, so
+ * 4 * F * K * Np <= 8388608 doubles (F * K * Np <= 2097152; larger sizes
+ * return DGC_ERR_INVALID). Allocates and synchronises nothing; asynchronous
+ * on the context's stream. Calls that overlap on one GPU (several streams,
+ * contexts or threads) share the array and race. */
+int dgc_rk4_step_noalloc(dgc_op_t op, double* const* inout, size_t size, double t, double dt);
+
 /* Time loop t0 -> t_end with step dt, the last step truncated to land on
  * t_end (examples/maxwell_2d/main.cpp:199-209). Full steps are replayed from a
  * captured hipGraph. *t_out / *steps_out report the final time and the step
