@@ -14,10 +14,11 @@ and Maxwell TM at N = 1..3 on 2048 x 2048 squares, K = 8.4M: working sets of
 0.8-6 GB, far above the 256 MB Infinity Cache), the reference CPU path
 timed on a bounded sample on the host cores, and the IPDG Poisson solvers'
 time to FP64 relative residual 1e-10 (block-Jacobi PCG, two-level PCG and
-flexible CG with the learned preconditioner) at K = 32768 N = 3 and
-K = 131072 N = 2 over held-out right-hand sides.
+flexible CG with the learned preconditioner, trained on 4096 pairs from the
+solver's own iterates) at K = 32768 N = 3 and K = 131072 N = 2 over 256
+held-out right-hand sides.
 
-    python bench.py [--steps 200] [--warmup 10] [--gpus 1] [--ipdg-full]
+    python bench.py [--steps 200] [--warmup 10] [--gpus 1] [--ipdg-quick]
 """
 from __future__ import annotations
 
@@ -284,14 +285,18 @@ def maxwell_c1(dgc, ctx, with_cpu=True):
     return out
 
 
-def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
+def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, n_eval_bj, mixed=False):
     """IPDG Poisson -Lap u = f on [-1,1]^2, u = 0 on the boundary, tau = (N+1)(N+2)/h,
     n_side x n_side squares split into triangles, jitter 0.1h (seed 3). Time to
-    FP64 relative residual 1e-10 for block-Jacobi PCG, two-level PCG (gx x gx
-    aggregates) and flexible CG with the learned preconditioner on the two-level
-    base, over held-out right-hand sides b = M f (f: 2 pi^2 sin sin + 16 Fourier
-    modes with k^-2 amplitudes, seeds 10000..). The network is trained first on
-    n_train pairs r = S e from Gaussian random fields (seeds 100..)."""
+    FP64 relative residual 1e-10 for block-Jacobi PCG (the first n_eval_bj
+    right-hand sides), two-level PCG (gx x gx aggregates) and flexible CG with
+    the learned preconditioner on the two-level base (all n_eval), over held-out
+    right-hand sides b = M f (f: 2 pi^2 sin sin + 16 Fourier modes with k^-2
+    amplitudes, seeds 10000..). The network is first trained on n_train
+    (residual, error) pairs produced by the solver itself: 16 recorded iterates
+    (every 16 iterations) of two-level PCG solves of n_train / 16 Gaussian
+    random-field right-hand sides (k^-2 amplitudes, seeds 100..); the last
+    sixteenth of the pairs (whole solves) is held out."""
     mesh = dgc.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=3)
     disc = dgc.Discretization(ctx, mesh, N)
     op = dgc.IPDG(disc, tau_mode=1)
@@ -303,15 +308,15 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
     t_coarse = time.perf_counter() - t0
     net = dgc.LearnedPreconditioner(op, seed=0, base=2)
     t0 = time.perf_counter()
-    net.build_dataset(mesh, n_train, seed0=100, modes=16)
+    pairs = net.build_dataset_solver(mesh, n_train // 16, seed0=100, modes=16, snapshots=16)
     t_data = time.perf_counter() - t0
-    held = max(1, n_train // 16)
-    loss_before = net.eval_loss(n_train - held, held)
+    held = max(1, pairs // 16)
+    loss_before = net.eval_loss(pairs - held, held)
     t0 = time.perf_counter()
-    hist = net.train(0, n_train - held, epochs=epochs, lr=3e-4, seed=0)
+    hist = net.train(0, pairs - held, epochs=epochs, lr=3e-4, seed=0)
     t_train = time.perf_counter() - t0
-    loss_after = net.eval_loss(n_train - held, held)
-    print(f"# IPDG K={disc.K} N={N}: coarse {gx}x{gx} ({nc} unknowns) {t_coarse:.2f} s, dataset {t_data:.1f} s, "
+    loss_after = net.eval_loss(pairs - held, held)
+    print(f"# IPDG K={disc.K} N={N}: coarse {gx}x{gx} ({nc} unknowns) {t_coarse:.2f} s, {pairs} pairs in {t_data:.1f} s, "
           f"training {t_train:.1f} s, held-out energy loss {loss_before:.4f} -> {loss_after:.4f}",
           file=sys.stderr, flush=True)
     xg, yg = mesh.dof_coords(N, node_major=True)
@@ -324,8 +329,10 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
         op.mass(dgc.DeviceArray.from_host(ctx, f), b)
         runs = (("bj_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=1)),
                 ("two_level_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=2)),
-                ("learned_fcg", lambda: net.fcg(b, x, rtol=1e-10)))
+                ("learned_fcg", lambda: net.fcg(b, x, rtol=1e-10, stall_window=40, stall_factor=0.5)))
         for name, fn in runs:
+            if name == "bj_pcg" and i >= n_eval_bj:
+                continue
             if i == 0:
                 fn()  # graph capture and first-touch outside the timing
             r = fn()
@@ -337,16 +344,19 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, mixed=False):
     out = dict(K=disc.K, N=N, n=disc.size, coarse_grid=[gx, gx], coarse_unknowns=nc,
                setup_s=dict(block_jacobi=round(t_bj, 3), coarse=round(t_coarse, 3), dataset=round(t_data, 2),
                             training=round(t_train, 2)),
-               training=dict(pairs=n_train - held, held_out=held, epochs=epochs,
+               training=dict(pairs=pairs - held, held_out=held, solves=n_train // 16, epochs=epochs,
+                             rhs_seeds=[100, 100 + n_train // 16 - 1], snapshots_per_solve=16,
                              epoch_loss=[round(v, 5) for v in hist], held_out_loss_untrained=round(loss_before, 5),
                              held_out_loss_trained=round(loss_after, 5)),
-               eval_rhs=dict(count=n_eval, seeds=[10000, 10000 + n_eval - 1]), fcg_stall_fallbacks=fallbacks)
+               eval_rhs=dict(count=n_eval, seeds=[10000, 10000 + n_eval - 1], bj_pcg_count=min(n_eval, n_eval_bj)),
+               fcg_stall_fallbacks=fallbacks)
     for name, v in res.items():
         out[name] = dict(median_iters=int(np.median(v["iters"])), median_ms=round(float(np.median(v["ms"])), 3),
                          mean_ms=round(float(np.mean(v["ms"])), 3), max_relres=float(np.max(v["relres"])))
     out["speedup_learned_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["learned_fcg"]["median_ms"], 2)
     out["speedup_two_level_vs_bj"] = round(out["bj_pcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 2)
     out["learned_vs_two_level_ms"] = round(out["learned_fcg"]["median_ms"] / out["two_level_pcg"]["median_ms"], 3)
+    out["learned_vs_two_level_iters"] = round(out["learned_fcg"]["median_iters"] / out["two_level_pcg"]["median_iters"], 3)
     out["iteration_bandwidth"] = {
         name: cg_iteration_roofline(disc.K, N, out[name]["median_ms"], out[name]["median_iters"],
                                     nc if name == "two_level_pcg" else 0)
@@ -713,8 +723,9 @@ def main():
     ap.add_argument("--widened-only", action="store_true",
                     help="run only the gradient / divergence, Stokes and 1D advection timings (for kernel traces)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--ipdg-full", action="store_true",
-                    help="IPDG comparison at full size: 4096 training pairs, 256 held-out right-hand sides")
+    ap.add_argument("--ipdg-quick", action="store_true",
+                    help="IPDG comparison on a small sample (512 training pairs, 8 held-out right-hand sides) "
+                         "instead of the stated 4096 / 256")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -793,11 +804,15 @@ def main():
         extra["assemble"] = assemble_bench(dgc, ctx)
 
     if not args.no_ipdg:
-        full = args.ipdg_full
+        quick = args.ipdg_quick
         extra["ipdg_apply"] = ipdg_apply_bench(dgc, ctx)
+        # the learned-preconditioner configuration: 4096 training pairs, 256
+        # held-out right-hand sides (block-Jacobi PCG on the first 32 of them)
         extra["ipdg"] = [
-            ipdg_bench(dgc, ctx, 128, 3, 32, 4096 if full else 1024, 1, 256 if full else 24, mixed=True),
-            ipdg_bench(dgc, ctx, 256, 2, 32, 4096 if full else 512, 1, 256 if full else 6),
+            ipdg_bench(dgc, ctx, 128, 3, 32, 512 if quick else 4096, 1 if quick else 3, 8 if quick else 256,
+                       8 if quick else 32, mixed=True),
+            ipdg_bench(dgc, ctx, 256, 2, 32, 512 if quick else 4096, 1 if quick else 3, 8 if quick else 256,
+                       8 if quick else 32),
         ]
 
     if not args.no_ensemble:

@@ -73,6 +73,7 @@ EnsFn stage_of(int stage) {
 
 template <bool NT>
 EnsFn ensemble_fn(int order, int wt, int stage) {
+  if (wt != kMfmaTiling && wt != 1 && wt != 2 && wt != 4) return nullptr;  // 256-thread workgroups only
   switch (order * 8 + wt) {
     case 1 * 8 + 2: return stage_of<1, 2, NT>(stage);
     case 1 * 8 + 4: return stage_of<1, 4, NT>(stage);

@@ -126,7 +126,18 @@ __global__ void sum256_kernel(const double* __restrict__ partial, double* out) {
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+__global__ void axpby_dev_kernel(double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                                 double* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
 }  // namespace
+
+hipError_t axpby_device(double a, const double* x, double b, const double* y, double* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(axpby_dev_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, a, x, b, y, out, n);
+  return hipGetLastError();
+}
 
 hipError_t dot_device(const double* a, const double* b, size_t n, double* partial, double* out, hipStream_t s) {
   hipLaunchKernelGGL(dot_kernel, dim3(256), dim3(256), 0, s, a, b, n, partial);

@@ -119,17 +119,24 @@ hipError_t launch_ensemble_stage(int order, int wt, bool streaming, int stage, c
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }
 
+// Tiles per workgroup and workgroup size of the FP64 stage kernels: with
+// wt = 1, 2 or 4 waves per tile a 256-thread workgroup holds 4/wt tiles; with
+// wt = 8 or 16 one tile has a 512- or 1024-thread workgroup to itself (fewer
+// output rows per wave: a shorter critical path on meshes of a few tiles).
+constexpr int tiles_per_group(int wt) { return wt >= kWaves ? 1 : kWaves / wt; }
+constexpr int group_threads(int wt) { return kTile * (wt > kWaves ? wt : kWaves); }
+
 // LDS (doubles) of one FP64 stage workgroup: node rows and face fluxes of its
-// 4/wt tiles.
+// tiles.
 constexpr int stage_lds_doubles(int N, int physics, int wt) {
-  return (kWaves / wt) * (physics == kMaxwellTM ? 3 : 1) * ((N + 1) * (N + 2) / 2 + 3 * (N + 1)) * kTile;
+  return tiles_per_group(wt) * (physics == kMaxwellTM ? 3 : 1) * ((N + 1) * (N + 2) / 2 + 3 * (N + 1)) * kTile;
 }
 
-// wt: waves per 64-cell tile (1, 2 or 4); a variant whose LDS would not fit
-// falls back to 4 (supported_waves_per_tile tells which one runs).
+// wt: waves per 64-cell tile (1, 2, 4, 8 or 16); a variant whose LDS would
+// not fit falls back to 4 (supported_waves_per_tile tells which one runs).
 hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s);
 int supported_waves_per_tile(int order, int physics, int wt);
-int default_waves_per_tile(int order, int physics);
+int default_waves_per_tile(int order, int physics, int K);
 
 }  // namespace dgc
 

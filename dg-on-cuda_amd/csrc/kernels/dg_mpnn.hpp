@@ -106,6 +106,8 @@ hipError_t fcg_scalars(const double* partial, int nparts, int which, double* sca
 hipError_t grf_eval(const double* x, const double* y, const double* amp, int M, size_t n, double* e, hipStream_t s);
 hipError_t mpnn_target(int K, int Np, const double* e, const double* zb, const double* mscal, float* t,
                        hipStream_t s);
+// out = a x + b y over n doubles
+hipError_t axpby_device(double a, const double* x, double b, const double* y, double* out, size_t n, hipStream_t s);
 // out[0] = a . b over n doubles (fixed-order two-level reduction; partial holds >= 256 doubles)
 hipError_t dot_device(const double* a, const double* b, size_t n, double* partial, double* out, hipStream_t s);
 

@@ -8,7 +8,7 @@
 namespace dgc {
 
 template <int N, int PHYS, int STAGE, int WT, bool NT>
-__global__ __launch_bounds__(kBlock) void dg_stage_kernel(const StageParams p) {
+__global__ __launch_bounds__(group_threads(WT)) void dg_stage_kernel(const StageParams p) {
   __shared__ double lds[stage_lds_doubles(N, PHYS, WT)];
   stage_tile<N, PHYS, STAGE, WT, NT>(p, xcd_group(blockIdx.x, gridDim.x), lds);
 }
@@ -26,11 +26,10 @@ constexpr StageFn stage_fn(int stage) {
 
 // A variant is built only if its LDS (all tiles of the workgroup) stays
 // within 80 KB, so that at least two workgroups share a CU; larger requests
-// fall back to one tile per workgroup (WT = 4).
+// fall back to one tile per workgroup (WT = 4). WT = 8, 16 hold one tile.
 template <int N, int PHYS, int WT>
 constexpr bool fits() {
-  constexpr int Np = Tri<N>::Np, NF = 3 * Tri<N>::Nfp, F = PHYS == kMaxwellTM ? 3 : 1;
-  return WT == 4 || (kWaves / WT) * F * (Np + NF) * kTile * 8 <= 80 * 1024;
+  return WT >= 4 || stage_lds_doubles(N, PHYS, WT) * 8 <= 80 * 1024;
 }
 
 template <int N, int PHYS, bool NT>
@@ -41,6 +40,8 @@ StageFn pick_wt(int wt, int stage) {
   if constexpr (fits<N, PHYS, 2>()) {
     if (wt == 2) return stage_fn<N, PHYS, 2, NT>(stage);
   }
+  if (wt == 8) return stage_fn<N, PHYS, 8, NT>(stage);
+  if (wt == 16) return stage_fn<N, PHYS, 16, NT>(stage);
   return stage_fn<N, PHYS, 4, NT>(stage);
 }
 

@@ -7,8 +7,9 @@
 //   src/explicit_runge_kutta.h:33-51 + src/axpy.h  (6 axpy passes per step, fused here)
 //
 // Mapping. A tile is 64 consecutive cells (lane = cell) served by WT waves
-// (WT = 1, 2 or 4, chosen per order by the autotuner); a 256-thread
-// workgroup holds 4/WT tiles, and workgroups are renumbered so that each XCD
+// (WT = 1, 2 or 4, chosen per order by the autotuner, and 8 or 16 for meshes
+// of a few tiles); a 256-thread workgroup holds 4/WT tiles (WT = 8, 16: one
+// tile per 512- or 1024-thread workgroup), and workgroups are renumbered so that each XCD
 // (own L2) sweeps a contiguous range of tiles, keeping neighbour traces in
 // the L2 that loaded them.
 //
@@ -38,7 +39,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   using T = Tri<N>;
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
   constexpr int F = PHYS == kMaxwellTM ? 3 : 1;
-  constexpr int TPW = kWaves / WT;        // tiles per workgroup
+  constexpr int TPW = tiles_per_group(WT);  // tiles per workgroup
   constexpr int RPW = (Np + WT - 1) / WT;  // output rows per wave (contiguous block)
   constexpr int LPW = (Np + WT - 1) / WT;  // input rows loaded per wave (strided)
   constexpr int FPW = (NF + WT - 1) / WT;  // face slots per wave (strided)
@@ -166,6 +167,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
 
   // ---- 3. stage operands of this wave's rows (in flight during the FMAs) --
   const int j0 = wt * RPW;
+  if (j0 >= Np) return;  // a wave without output rows (no barrier follows)
   double o1[F][RPW], o2[F][RPW], o3[F][RPW];
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {

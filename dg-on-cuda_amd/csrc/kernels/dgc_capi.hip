@@ -357,7 +357,7 @@ static int create_op(dgc_disc_t disc, int physics, const double* Dr, const doubl
   op->ax = ax;
   op->ay = ay;
   op->bc = bc;
-  op->wt = dgc::default_waves_per_tile(disc->order, physics);
+  op->wt = dgc::default_waves_per_tile(disc->order, physics, disc->K);
   int rc = upload_operators(op, Dr, Ds, lift);
   if (rc == DGC_OK) rc = build_mfma_operands(op);
   if (rc != DGC_OK) {
@@ -489,40 +489,50 @@ int dgc_axpy(dgc_ctx_t ctx, double a, const double* x, size_t n, const double* y
 // One fused RK4 step. wk mapping: wk0 = stage accumulator (k1 + 2 k2),
 // wk1 / wk2 = stage inputs (ping-pong), wk3 = k3. wk4 is unused.
 // With `evs` (8 events) each stage launch is bracketed by evs[2s], evs[2s+1].
-static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, double* const* wk,
-                        bool use_clock, hipEvent_t* evs = nullptr) {
+// The four stage parameter records of one step (stages 1..4 in p[0..3]).
+static void step_params(dgc_op_s* op, double* const* u, double t, double dt, double* const* wk, bool use_clock,
+                        dgc::StageParams p[4]) {
   const int F = op->F;
-  dgc::StageParams p = base_params(op);
+  dgc::StageParams b = base_params(op);
   for (int f = 0; f < F; ++f) {
-    p.u[f] = u[f];
-    p.acc[f] = wk[0 * F + f];
-    p.k3[f] = wk[3 * F + f];
+    b.u[f] = u[f];
+    b.acc[f] = wk[0 * F + f];
+    b.k3[f] = wk[3 * F + f];
   }
   if (use_clock) {
-    p.tclock = op->tclock;
-    p.dt_clock = dt;
+    b.tclock = op->tclock;
+    b.dt_clock = dt;
   }
   const double half = 0.5 * dt, sixth = dt / 6.0;
   struct Stage {
-    int id;
     double tfrac;
     double c1;
     int in_buf;   // -1: state, else work index
     int out_buf;  // -1: state
-  } stages[4] = {{1, 0.0, half, -1, 1}, {2, 0.5, half, 1, 2}, {3, 0.5, dt, 2, 1}, {4, 1.0, sixth, 1, -1}};
-  for (const Stage& s : stages) {
-    p.t = (s.tfrac == 0.0) ? t : (s.tfrac == 1.0 ? t + dt : t + 0.5 * dt);
-    p.tfrac = s.tfrac;
-    p.c1 = s.c1;
+  } stages[4] = {{0.0, half, -1, 1}, {0.5, half, 1, 2}, {0.5, dt, 2, 1}, {1.0, sixth, 1, -1}};
+  for (int i = 0; i < 4; ++i) {
+    const Stage& s = stages[i];
+    p[i] = b;
+    p[i].t = (s.tfrac == 0.0) ? t : (s.tfrac == 1.0 ? t + dt : t + 0.5 * dt);
+    p[i].tfrac = s.tfrac;
+    p[i].c1 = s.c1;
     for (int f = 0; f < F; ++f) {
-      p.in[f] = s.in_buf < 0 ? u[f] : wk[s.in_buf * F + f];
-      p.out[f] = s.out_buf < 0 ? u[f] : wk[s.out_buf * F + f];
+      p[i].in[f] = s.in_buf < 0 ? u[f] : wk[s.in_buf * F + f];
+      p[i].out[f] = s.out_buf < 0 ? u[f] : wk[s.out_buf * F + f];
     }
-    hipStream_t st = op->disc->ctx->stream;
-    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1)], st));
-    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, s.id, op->wt, p, st);
+  }
+}
+
+static int enqueue_step(dgc_op_s* op, double* const* u, double t, double dt, double* const* wk,
+                        bool use_clock, hipEvent_t* evs = nullptr) {
+  dgc::StageParams p[4];
+  step_params(op, u, t, dt, wk, use_clock, p);
+  hipStream_t st = op->disc->ctx->stream;
+  for (int i = 0; i < 4; ++i) {
+    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * i], st));
+    hipError_t e = dgc::launch_stage(op->disc->order, op->physics, i + 1, op->wt, p[i], st);
     if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, "failed to launch kernel/device code!");
-    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * (s.id - 1) + 1], st));
+    if (evs) DGC_HIP_TRY(hipEventRecord(evs[2 * i + 1], st));
   }
   return DGC_OK;
 }
@@ -702,7 +712,7 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
 
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_waves_per_tile: null operator");
-  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == dgc::kMfmaTiling,
+  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == 8 || wt == 16 || wt == dgc::kMfmaTiling,
               "dgc_op_set_waves_per_tile: wt must be 1, 2, 4 or 0 (FP64 matrix-core variant)");
   op->wt = dgc::supported_waves_per_tile(op->disc->order, op->physics, wt);
   if (op->graph) {
@@ -713,6 +723,8 @@ int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
 }
 
 int dgc_op_waves_per_tile(dgc_op_t op) { return op ? op->wt : 0; }
+
+
 
 int dgc_op_set_precision(dgc_op_t op, int precision) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_precision: null operator");
@@ -768,7 +780,11 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
     }
   }
   hipStream_t st = op->disc->ctx->stream;
-  const int cand[4] = {1, 2, 4, dgc::kMfmaTiling};
+  // 8 and 16 waves per tile (one tile per workgroup) only pay on meshes of a
+  // few tiles; their times are not reported in ms_out
+  const int tiles = (op->disc->K + dgc::kTile - 1) / dgc::kTile;
+  const int ncand = tiles <= 256 ? 6 : 4;
+  const int cand[6] = {1, 2, 4, dgc::kMfmaTiling, 8, 16};
   float best = 0.f;
   int best_wt = op->wt;
   hipEvent_t e0, e1, e2;
@@ -776,9 +792,9 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
   DGC_HIP_TRY(hipEventCreate(&e1));
   DGC_HIP_TRY(hipEventCreate(&e2));
   const int order = op->disc->order, phys = op->physics;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < ncand; ++i) {
     const int wt = cand[i];
-    if (ms_out) ms_out[i] = -1.f;
+    if (ms_out && i < 4) ms_out[i] = -1.f;
     if (dgc::supported_waves_per_tile(order, phys, wt) != wt) continue;
     if (op->precision == 1 && wt != 4) continue;  // the matrix-core kernel has one tiling
     // warm-up step, then reps launches of stage 2, then reps steps
@@ -801,7 +817,7 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
       return dgc_set_error(DGC_ERR_LAUNCH, std::string("dgc_op_autotune: ") + hipGetErrorString(e));
     }
     ms2 /= reps;
-    if (ms_out) ms_out[i] = ms2;
+    if (ms_out && i < 4) ms_out[i] = ms2;
     const float step = ms_step / reps;
     if (best == 0.f || step < best) {
       best = step;

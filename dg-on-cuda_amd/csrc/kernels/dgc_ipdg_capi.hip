@@ -359,12 +359,25 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
   // after whole chunks the current scalars are in record 0 (fused) or L.scal
   const T* cur = fused_iteration(s) ? L.rec : L.scal;
   T h[dgc::kCgScalars];
+  const bool snapping = std::is_same<T, double>::value && s->snap != nullptr;
+  const size_t n = size_t(s->disc->K) * s->disc->Np;
+  bool snapped = false;  // an iterate was recorded after the last replay
   for (;;) {
     DGC_HIP_TRY(hipMemcpyAsync(h, cur, sizeof(h), hipMemcpyDeviceToHost, st));
     DGC_HIP_TRY(hipStreamSynchronize(st));
-    if (h[dgc::kDone] != T(0)) break;
+    if (h[dgc::kDone] != T(0)) {
+      if (snapped) --s->snap_taken;  // recorded after (or at) convergence
+      break;
+    }
     if (int(h[dgc::kIter]) >= maxit) break;
     DGC_HIP_TRY(hipGraphLaunch(L.graph, st));
+    snapped = false;
+    if (snapping && s->snap_taken < s->nsnap) {
+      DGC_HIP_TRY(hipMemcpyAsync(s->snap + size_t(s->snap_taken) * n, L.x, n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
+      ++s->snap_taken;
+      snapped = true;
+    }
   }
   *iters = int(h[dgc::kIter]);
   return DGC_OK;
@@ -967,6 +980,7 @@ int dgc_ipdg_pcg(dgc_ipdg_t s, const double* b, double* x, double rtol, int maxi
     rc = cg_init<double>(s, L, b, rtol * rtol * bb, maxit - total, restart == 0, s->hist + total);
     int it = 0;
     if (rc == DGC_OK) rc = cg_run<double>(s, L, s->hist + total, maxit - total, &it);
+    s->snap = nullptr;  // restarts record nothing
     total += it;
     if (rc != DGC_OK) break;
     // true residual r = b - S x

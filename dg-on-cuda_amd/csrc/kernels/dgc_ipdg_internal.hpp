@@ -63,6 +63,13 @@ struct dgc_ipdg_s {
   std::vector<void*> coarse_owned;
   std::vector<double> coarse_A;  // Phi^T S Phi (host copy, row-major)
   int coarse_gen = 0;            // bumped whenever arrays that solver graphs capture change
+  // Iterates recorded by the FP64 PCG for the learned preconditioner's training
+  // data: with snap set, the first run of a solve copies x after every
+  // graph-replayed chunk of iterations into snap[taken * n] while taken < nsnap
+  // (a chunk that finished the solve is not kept).
+  double* snap = nullptr;
+  int nsnap = 0;
+  int snap_taken = 0;
 };
 
 // z += Phi A_c^{-1} Phi^T r with partials (r.z, r.r) or (z.z, r.r) into `partial`.

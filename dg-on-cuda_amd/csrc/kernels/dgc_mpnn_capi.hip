@@ -261,6 +261,34 @@ int precondition(dgc_mpnn_s* m, const double* r, const double* zb, const double*
   return DGC_OK;
 }
 
+// Target scale c_t = RMS((e - zb) / s_z) of one training pair (e in m->e,
+// r = S e): the typical size of what the network's output has to supply.
+int set_target_scale_from(dgc_mpnn_s* m, const double* r) {
+  hipStream_t st = stream_of(m);
+  const size_t n = size_t(m->K) * m->Np;
+  IpdgLevel<double>& Lv = m->s->f64;
+  double* zb = Lv.z;
+  int rc = base_apply(m, r, zb);
+  if (rc) return rc;
+  DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, double(n), nullptr, st));
+  std::vector<double> he(n), hz(n), ms(dgc::kMpnnScalars);
+  DGC_HIP_TRY(hipMemcpyAsync(he.data(), m->e, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipMemcpyAsync(hz.data(), zb, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipMemcpyAsync(ms.data(), m->mscal, ms.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  double acc = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double d = (he[i] - hz[i]) / ms[dgc::kSz];
+    acc += d * d;
+  }
+  m->ct = std::sqrt(acc / double(n));
+  DGC_REQUIRE(std::isfinite(m->ct) && m->ct > 0, "training data: degenerate target scale");
+  ms[dgc::kCt] = m->ct;
+  DGC_HIP_TRY(hipMemcpyAsync(m->mscal, ms.data(), ms.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  return DGC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -454,7 +482,6 @@ int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_
   DGC_HIP_TRY(hipMalloc(&m->ds_r, size_t(nsamples) * n * sizeof(double)));
   m->ds_er.assign(nsamples, 0.0);
   IpdgLevel<double>& Lv = m->s->f64;
-  double* zb = Lv.z;
   std::vector<double> amp(size_t(modes) * modes);
   for (int smp = 0; smp < nsamples; ++smp) {
     // error field: sine modes of [-1,1]^2 with N(0,1) k^-2 amplitudes
@@ -472,32 +499,100 @@ int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_
     DGC_LAUNCH(dgc::dot_device(m->e, r, n, m->dotpart, m->dotpart + 256, st));
     DGC_HIP_TRY(hipMemcpyAsync(&m->ds_er[smp], m->dotpart + 256, sizeof(double), hipMemcpyDeviceToHost, st));
     if (smp == 0) {
-      // zb (the base step) and the input scales for the target-scale estimate
-      int rc = base_apply(m, r, zb);
+      int rc = set_target_scale_from(m, r);
       if (rc) return rc;
-      DGC_LAUNCH(dgc::fcg_scalars(Lv.partial, m->cg_grid, 3, m->tmp_scal, m->mscal, double(n), nullptr, st));
-      // target scale c_t: RMS of (e - zb) / s_z on the first sample
-      std::vector<double> he(n), hz(n), ms(dgc::kMpnnScalars);
-      DGC_HIP_TRY(hipMemcpyAsync(he.data(), m->e, n * sizeof(double), hipMemcpyDeviceToHost, st));
-      DGC_HIP_TRY(hipMemcpyAsync(hz.data(), zb, n * sizeof(double), hipMemcpyDeviceToHost, st));
-      DGC_HIP_TRY(hipMemcpyAsync(ms.data(), m->mscal, ms.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-      DGC_HIP_TRY(hipStreamSynchronize(st));
-      double acc = 0.0;
-      for (size_t i = 0; i < n; ++i) {
-        const double d = (he[i] - hz[i]) / ms[dgc::kSz];
-        acc += d * d;
-      }
-      m->ct = std::sqrt(acc / double(n));
-      DGC_REQUIRE(std::isfinite(m->ct) && m->ct > 0, "dgc_mpnn_build_dataset: degenerate target scale");
-      ms[dgc::kCt] = m->ct;
-      DGC_HIP_TRY(hipMemcpyAsync(m->mscal, ms.data(), ms.size() * sizeof(double), hipMemcpyHostToDevice, st));
-      DGC_HIP_TRY(hipStreamSynchronize(st));
     }
   }
   DGC_HIP_TRY(hipStreamSynchronize(st));
   for (int smp = 0; smp < nsamples; ++smp)
     DGC_REQUIRE(std::isfinite(m->ds_er[smp]) && m->ds_er[smp] > 0, "dgc_mpnn_build_dataset: e.r is not positive");
   m->nsamples = nsamples;
+  return DGC_OK;
+}
+
+// Training pairs from the solver's own iterations: for right-hand side i
+// (b = M f, f the Gaussian random field of seed seed0 + i), PCG with the
+// network's base preconditioner runs to 1e-11, recording its iterate x_j
+// after every 16 iterations (x_0 = 0); each recorded iterate gives the pair
+// e = x* - x_j, r = S e (the residual the preconditioner meets at that point
+// of a solve). *npairs receives the number of pairs (up to nrhs * snapshots;
+// a solve that converges early contributes fewer).
+int dgc_mpnn_build_dataset_solver(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nrhs,
+                                  uint64_t seed0, int modes, int snapshots, int* npairs) {
+  DGC_REQUIRE(m && dof_x && dof_y, "dgc_mpnn_build_dataset_solver: null argument");
+  DGC_REQUIRE(nrhs > 0 && snapshots > 0 && modes >= 1 && modes <= 32,
+              "dgc_mpnn_build_dataset_solver: bad nrhs, snapshots or modes");
+  dgc_ipdg_s* s = m->s;
+  DGC_REQUIRE(m->base == 1 || s->coarse.nc > 0, "dgc_mpnn_build_dataset_solver: the coarse space was released");
+  hipStream_t st = stream_of(m);
+  const size_t n = size_t(m->K) * m->Np;
+  if (!m->dof_x) {
+    DGC_HIP_TRY(hipMalloc(&m->dof_x, n * sizeof(double)));
+    DGC_HIP_TRY(hipMalloc(&m->dof_y, n * sizeof(double)));
+    DGC_HIP_TRY(hipMalloc(&m->e, n * sizeof(double)));
+  }
+  DGC_HIP_TRY(hipMemcpy(m->dof_x, dof_x, n * sizeof(double), hipMemcpyHostToDevice));
+  DGC_HIP_TRY(hipMemcpy(m->dof_y, dof_y, n * sizeof(double), hipMemcpyHostToDevice));
+  (void)hipFree(m->ds_r);
+  m->ds_r = nullptr;
+  m->nsamples = 0;
+  const size_t cap = size_t(nrhs) * snapshots;
+  DGC_HIP_TRY(hipMalloc(&m->ds_r, cap * n * sizeof(double)));
+  m->ds_er.assign(cap, 0.0);
+  double *snap = nullptr, *b = nullptr, *xs = nullptr;
+  struct Buffers {
+    double** p[3];
+    ~Buffers() {
+      for (double** q : p) (void)hipFree(*q);
+    }
+  } bufs{{&snap, &b, &xs}};
+  DGC_HIP_TRY(hipMalloc(&snap, size_t(snapshots) * n * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&b, n * sizeof(double)));
+  DGC_HIP_TRY(hipMalloc(&xs, n * sizeof(double)));
+  IpdgLevel<double>& Lv = s->f64;
+  std::vector<double> amp(size_t(modes) * modes);
+  size_t count = 0;
+  for (int i = 0; i < nrhs; ++i) {
+    Gauss g(seed0 + uint64_t(i));
+    for (int a = 1; a <= modes; ++a)
+      for (int c = 1; c <= modes; ++c) amp[size_t(a - 1) * modes + c - 1] = g() / double(a * a + c * c);
+    DGC_HIP_TRY(hipMemcpyAsync(m->amp, amp.data(), amp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    DGC_LAUNCH(dgc::grf_eval(m->dof_x, m->dof_y, m->amp, modes, n, m->e, st));
+    int rc = dgc_ipdg_mass(s, m->e, b);
+    if (rc) return rc;
+    DGC_HIP_TRY(hipMemsetAsync(snap, 0, n * sizeof(double), st));  // x_0 = 0
+    s->snap = snap + n;
+    s->nsnap = snapshots - 1;
+    s->snap_taken = 0;
+    int it = 0;
+    double rel = 0.0;
+    float ms = 0.f;
+    rc = dgc_ipdg_pcg(s, b, xs, 1e-11, 20000, m->base, &it, &rel, nullptr, &ms);
+    const int taken = 1 + s->snap_taken;
+    s->snap = nullptr;
+    s->nsnap = s->snap_taken = 0;
+    if (rc == DGC_ERR_NOT_CONVERGED && rel <= 1e-10) rc = DGC_OK;  // x* to 1e-10 is enough
+    if (rc) return rc;
+    for (int j = 0; j < taken; ++j) {
+      // e = x* - x_j, r = S e, e.r
+      double* r = m->ds_r + count * n;
+      DGC_LAUNCH(dgc::axpby_device(1.0, xs, -1.0, snap + size_t(j) * n, m->e, n, st));
+      dgc::IpdgParams<double> pr = Lv.params(*s);
+      pr.u = m->e;
+      pr.out = r;
+      DGC_LAUNCH(dgc::ipdg_apply<double>(m->N, pr, dgc::kSpd, st));
+      DGC_LAUNCH(dgc::dot_device(m->e, r, n, m->dotpart, m->dotpart + 256, st));
+      DGC_HIP_TRY(hipMemcpyAsync(&m->ds_er[count], m->dotpart + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+      if (count == 0 && (rc = set_target_scale_from(m, r))) return rc;
+      ++count;
+    }
+  }
+  DGC_HIP_TRY(hipStreamSynchronize(st));
+  for (size_t k = 0; k < count; ++k)
+    DGC_REQUIRE(std::isfinite(m->ds_er[k]) && m->ds_er[k] > 0, "dgc_mpnn_build_dataset_solver: e.r is not positive");
+  m->ds_er.resize(count);
+  m->nsamples = int(count);
+  if (npairs) *npairs = int(count);
   return DGC_OK;
 }
 

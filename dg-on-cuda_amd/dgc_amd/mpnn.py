@@ -31,6 +31,8 @@ def _declare():
     d("dgc_mpnn_eval", [_vp, _fp, _fp])
     d("dgc_mpnn_loss_grad", [_vp, _fp, _fp, _dp, _fp])
     d("dgc_mpnn_build_dataset", [_vp, _dp, _dp, C.c_int, C.c_uint64, C.c_int])
+    d("dgc_mpnn_build_dataset_solver", [_vp, _dp, _dp, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                                         C.POINTER(C.c_int)])
     d("dgc_mpnn_train", [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp])
     d("dgc_mpnn_eval_loss", [_vp, C.c_int, C.c_int, _dp])
     d("dgc_mpnn_target_scale", [_vp, _dp])
@@ -108,6 +110,17 @@ class LearnedPreconditioner:
         check(lib().dgc_mpnn_build_dataset(self._h, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), nsamples,
                                            seed0, modes))
         self.nsamples = nsamples
+
+    def build_dataset_solver(self, mesh, nrhs: int, seed0: int = 100, modes: int = 16,
+                             snapshots: int = 16) -> int:
+        """Pairs (r = S e, e = x* - x_j) from the iterates of base-preconditioned
+        PCG solves of nrhs random right-hand sides; returns the pair count."""
+        x, y = mesh.dof_coords(self.ipdg.disc.order, node_major=True)
+        n = C.c_int()
+        check(lib().dgc_mpnn_build_dataset_solver(self._h, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), nrhs,
+                                                  seed0, modes, snapshots, C.byref(n)))
+        self.nsamples = n.value
+        return n.value
 
     def train(self, first: int, count: int, epochs: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         hist = np.zeros(epochs)

@@ -234,15 +234,18 @@ int dgc_advection_create(dgc_disc_t disc, const double* Dr, const double* Ds, co
 int dgc_op_destroy(dgc_op_t op);
 int dgc_op_num_fields(dgc_op_t op);
 
-/* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4; the
- * reference's launch-shape arguments, main.cpp argv[2], play this role), or
- * wt = 0 for the FP64 matrix-core variant (Dr u, Ds u and LIFT flux as
- * 16x16x4 f64 MFMA products, one 64-cell tile per workgroup; FP64 throughout).
- * A tiling whose LDS would not fit falls back to 4. dgc_op_autotune times an
- * RK4 step (the 4 stage kernels) of every built tiling on the caller's buffers
- * (work vectors only are written: stage 4 updates wk2 in place of the state)
- * and keeps the fastest; ms_out[4] receives the mean stage-2 kernel time of
- * wt = 1, 2, 4, 0 (-1 where not built). No reference counterpart. */
+/* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4: 4/wt
+ * tiles per 256-thread workgroup; 8 or 16: one tile per 512- or 1024-thread
+ * workgroup, for meshes of a few tiles; the reference's launch-shape
+ * arguments, main.cpp argv[2], play this role), or wt = 0 for the FP64
+ * matrix-core variant (Dr u, Ds u and LIFT flux as 16x16x4 f64 MFMA
+ * products, one 64-cell tile per workgroup; FP64 throughout). A tiling whose
+ * LDS would not fit falls back to 4. dgc_op_autotune times an RK4 step (the 4
+ * stage kernels) of every built tiling (8 and 16 only on meshes of at most 256
+ * tiles) on the caller's buffers (work vectors only are written: stage 4
+ * updates wk2 in place of the state) and keeps the fastest; ms_out[4]
+ * receives the mean stage-2 kernel time of wt = 1, 2, 4, 0 (-1 where not
+ * built). No reference counterpart. */
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt);
 int dgc_op_waves_per_tile(dgc_op_t op);
 /* Arithmetic of the stage contractions (no reference counterpart).
@@ -441,6 +444,17 @@ int dgc_mpnn_loss_grad(dgc_mpnn_t m, const float* X, const float* T, double* los
  * Also sets the target scale c_t = RMS((e - P_BJ r) / s_z) of sample 0. */
 int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nsamples,
                            uint64_t seed0, int modes);
+/* Training pairs from the solver itself: for right-hand side i, b = M f with
+ * f the Gaussian random field of dgc_mpnn_build_dataset (modes^2 sine modes,
+ * N(0,1) (m^2+n^2)^-1 amplitudes, seed seed0 + i), PCG with the network's
+ * base preconditioner runs to 1e-11 while its iterate x_j is recorded every
+ * 16 iterations (x_0 = 0, up to `snapshots` per solve); each recorded iterate
+ * gives the pair e = x* - x_j, r = S e, the residual a preconditioner meets
+ * at that point of a solve. *npairs: the number of pairs (nrhs * snapshots
+ * unless a solve converges within 16 (snapshots - 1) iterations). Sets c_t
+ * from the first pair. */
+int dgc_mpnn_build_dataset_solver(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nrhs,
+                                  uint64_t seed0, int modes, int snapshots, int* npairs);
 int dgc_mpnn_target_scale(dgc_mpnn_t m, double* ct);
 int dgc_mpnn_set_target_scale(dgc_mpnn_t m, double ct);
 /* Energy-norm loss of one residual r = S e (device, node-major) with

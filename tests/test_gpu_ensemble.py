@@ -46,6 +46,9 @@ def test_ensemble_equals_per_instance_kernels_and_oracle(ctx):
     ens.steps(10)
     for k, (mesh, disc, op, u0, dt) in enumerate(inst):
         got = _download(states[k], disc.K, disc.Np)
+        # the per-instance kernels at the tiling the ensemble runs (the small
+        # instances' own default, 16 waves per tile, has no ensemble kernel)
+        op.waves_per_tile = info["waves_per_tile"][k]
         ref_state = _upload(ctx, u0, disc.K, disc.Np)
         op.rk4_steps(ref_state, 0.0, dt, 10, op.new_work())
         want = _download(ref_state, disc.K, disc.Np)

@@ -53,7 +53,7 @@ def test_forward_matches_oracle(ctx):
     want, _ = mo.forward(p, X.astype(np.float32), nbr, bf=True)
     err = np.linalg.norm(y - want) / np.linalg.norm(want)
     print(f"forward rel err {err:.2e}")
-    assert err < 1e-2
+    assert err < 1e-6
 
 
 def test_gradients_match_oracle(ctx):
@@ -63,7 +63,7 @@ def test_gradients_match_oracle(ctx):
     T = np.random.default_rng(9).standard_normal((net.K, net.Np)).astype(np.float32)
     loss, g = net.loss_grad(X.astype(np.float32), T)
     wl, wg, _ = mo.loss_and_grads(p, X.astype(np.float32), T, nbr, bf=True)
-    assert abs(loss - wl) <= 1e-3 * wl
+    assert abs(loss - wl) <= 1e-6 * wl
     off = 0
     for k in ("A1", "B1", "b1", "A2", "B2", "b2", "A3", "B3", "b3", "Wo", "bo"):
         n = p[k].size
@@ -71,7 +71,7 @@ def test_gradients_match_oracle(ctx):
         off += n
         err = np.linalg.norm(gg - ww) / max(np.linalg.norm(ww), 1e-30)
         print(f"grad {k}: rel err {err:.2e}")
-        assert err < 3e-2, k
+        assert err < 1e-5, k
 
 
 def test_energy_loss_gradients_match_oracle(ctx):
@@ -105,8 +105,8 @@ def test_energy_loss_gradients_match_oracle(ctx):
     z = dgc_amd.node_to_cell_major(zd.download(), K, Np).reshape(K, Np)
     dz = np.linalg.norm((z - zb) - s_z * ct * wy) / np.linalg.norm(s_z * ct * wy)
     print(f"energy loss {loss:.6f} oracle {wl:.6f}; correction rel err {dz:.2e}")
-    assert dz < 1e-2
-    assert abs(loss - wl) <= 1e-3 * abs(wl)
+    assert dz < 1e-6
+    assert abs(loss - wl) <= 1e-6 * abs(wl)
     off = 0
     for k in ("A1", "B1", "b1", "A2", "B2", "b2", "A3", "B3", "b3", "Wo", "bo"):
         n = p[k].size
@@ -114,7 +114,7 @@ def test_energy_loss_gradients_match_oracle(ctx):
         off += n
         err = np.linalg.norm(gg - ww) / max(np.linalg.norm(ww), 1e-30)
         print(f"energy grad {k}: rel err {err:.2e}")
-        assert err < 3e-2, k
+        assert err < 1e-5, k
 
 
 def _poisson(ctx, n_side, N, seed=3):
@@ -138,12 +138,14 @@ def _oracle_relres(mesh, disc, x_node, b_node):
     return np.linalg.norm(b - Sx) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("base", [1, 2])
-def test_train_and_flexible_cg(ctx, base):
+@pytest.mark.parametrize("base,N", [(1, 3), (2, 3), (2, 2)])
+def test_train_and_flexible_cg(ctx, base, N):
     """Untrained (zero output layer) FCG reproduces PCG with the base
     preconditioner; training lowers the held-out energy loss; FCG with the
-    trained network reaches the FP64 tolerance by the oracle's residual."""
-    mesh, disc, op, b = _poisson(ctx, 32, 3)
+    trained network reaches the FP64 tolerance by the oracle's residual
+    (N = 2 on the two-level base: the setting of the 131072-cell benchmark
+    leg, here on a 32 x 32 mesh)."""
+    mesh, disc, op, b = _poisson(ctx, 32, N)
     if base == 2:
         op.coarse_space(mesh, 8)
     net = dgc_amd.LearnedPreconditioner(op, seed=0, base=base)
@@ -164,3 +166,22 @@ def test_train_and_flexible_cg(ctx, base):
     assert abs(r0["iters"] - rp["iters"]) <= 2
     assert after < before
     assert r1["relres"] <= 1e-10 and rr <= 2e-10
+
+
+def test_training_pairs_from_the_solver_itself(ctx):
+    """dgc_mpnn_build_dataset_solver: pairs (r = S e, e = x* - x_j) from the
+    recorded iterates of base-preconditioned PCG solves: e.r is positive, and
+    the base preconditioner (the untrained network) leaves a larger share of
+    the error energy on a late iterate (the hard modes) than on the first one
+    (x_0 = 0, e = x*)."""
+    mesh, disc, op, _ = _poisson(ctx, 32, 3)
+    op.coarse_space(mesh, 8)
+    net = dgc_amd.LearnedPreconditioner(op, seed=0, base=2)
+    n = net.build_dataset_solver(mesh, 4, seed0=100, modes=16, snapshots=4)
+    assert n == 16
+    first = net.eval_loss(0, 1)
+    late = net.eval_loss(3, 1)
+    print(f"base energy ratio: first pair {first:.4f}, fourth pair {late:.4f}")
+    assert 0 < first < late < 1.0
+    hist = net.train(0, 12, epochs=2, lr=3e-4, seed=0)
+    assert np.all(np.isfinite(hist))

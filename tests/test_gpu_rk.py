@@ -73,6 +73,23 @@ def test_maxwell_cavity_t1_matches_reference(ctx, c1):
     assert np.mean((got - exact) ** 2) < 1e-8
 
 
+@pytest.mark.parametrize("wt", [4, 8, 16])
+def test_maxwell_cavity_t1_every_small_mesh_tiling(ctx, golden, wt):
+    """The cavity run at 4, 8 and 16 waves per tile (the default on this
+    8-tile mesh is 16): the golden t = 1 state to the FP64 bar."""
+    fix = golden("maxwell_c1.npz")
+    mesh = dgc_amd.Mesh.from_arrays(fix["vx"], fix["vy"], fix["etov"])
+    disc = dgc_amd.Discretization(ctx, mesh, int(fix["order"]))
+    op = dgc_amd.MaxwellTM(disc)
+    assert op.waves_per_tile == 16
+    op.waves_per_tile = wt
+    K, Np = disc.K, disc.Np
+    u0 = np.concatenate([np.zeros(2 * K * Np), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    state = upload_fields(ctx, u0, 3, K, Np)
+    t, steps = op.rk4_run(state, 0.0, 1.0, float(fix["dt"]), op.new_work())
+    assert steps == int(fix["steps"]) and rel(download_fields(state, K, Np), fix["final"]) < TOL
+
+
 def test_single_steps_equal_graph_replay(ctx, c1):
     fix, mesh, disc, op = c1
     K, Np = disc.K, disc.Np
@@ -179,6 +196,7 @@ def test_maxwell_cavity_t1_fp64_matrix_core_variant(ctx, c1):
     K, Np = disc.K, disc.Np
     n = K * Np
     u0 = np.concatenate([np.zeros(2 * n), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
+    default_wt = op.waves_per_tile
     op.waves_per_tile = 0
     try:
         state = upload_fields(ctx, u0, 3, K, Np)
@@ -187,7 +205,7 @@ def test_maxwell_cavity_t1_fp64_matrix_core_variant(ctx, c1):
         print(f"maxwell C1 t=1 (FP64 MFMA variant) rel L2 vs reference = {err:.3e}")
         assert steps == int(fix["steps"]) and err < TOL
     finally:
-        op.waves_per_tile = 4
+        op.waves_per_tile = default_wt
 
 
 @pytest.mark.parametrize("N", [1, 3, 7])
@@ -335,14 +353,14 @@ def test_bad_arguments_are_errors(ctx, c1):
 
 @pytest.mark.parametrize("N", [1, 2, 3, 5])
 def test_every_tiling_gives_identical_results(ctx, N):
-    """The stage kernels' tilings (waves per tile, and 0: the FP64
-    matrix-core variant) agree to rounding (the summation order and
+    """The stage kernels' tilings (1, 2, 4, 8 or 16 waves per tile, and 0: the
+    FP64 matrix-core variant) agree to rounding (the summation order and
     multiply-add contraction differ per variant)."""
     mesh = dgc_amd.Mesh.structured(20, 13, 0, 1, 0, 1, jitter=0.05, seed=4, periodic=True)
     disc = dgc_amd.Discretization(ctx, mesh, N)
     results = []
     for phys in ("adv", "mx"):
-        for wt in (1, 2, 4, 0):
+        for wt in (1, 2, 4, 0, 8, 16):
             op = (dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO) if phys == "adv"
                   else dgc_amd.MaxwellTM(disc))
             op.waves_per_tile = wt
@@ -393,7 +411,7 @@ def test_autotune_leaves_state_untouched(ctx):
     u0 = np.random.default_rng(0).standard_normal(disc.size)
     st = [dgc_amd.DeviceArray.from_host(ctx, u0)]
     res = op.autotune(st, op.new_work(), reps=3)
-    assert res["choice"] in (1, 2, 4, 0) and res["ms"] and 0 in res["ms"]
+    assert res["choice"] in (1, 2, 4, 0, 8, 16) and res["ms"] and 0 in res["ms"]
     np.testing.assert_array_equal(st[0].download(), u0)
 
 
