@@ -10,7 +10,7 @@ hipError_t launch_ensemble_stage(int order, int wt, bool streaming, int stage, c
   ens_detail::EnsFn fn = streaming ? ens_detail::ensemble_fn_streaming(order, wt, stage)
                                    : ens_detail::ensemble_fn<false>(order, wt, stage);
   if (fn == nullptr || stage < 1 || stage > 4 || nitems < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fn, dim3(nitems), dim3(kBlock), 0, s, params, items);
+  hipLaunchKernelGGL(fn, dim3(nitems), dim3(group_threads(wt)), 0, s, params, items);
   return hipGetLastError();
 }
 

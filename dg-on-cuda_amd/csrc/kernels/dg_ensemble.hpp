@@ -2,7 +2,8 @@
 // and order each) advance together, one RK4 stage at a time.
 //
 // One launch per order and stage covers every instance of that order: its
-// work list holds one item per (instance, group of 4/WT tiles of 64 cells),
+// work list holds one item per (instance, group of 4/WT tiles of 64 cells;
+// one tile at WT = 8),
 // the items of an instance contiguous, and each XCD takes a contiguous range
 // of the list, so the neighbour traces a tile reads were usually loaded into
 // the same L2 by the tile before it. Per item the workgroup runs the device
@@ -34,7 +35,7 @@ using ConstParams = const StageParams __attribute__((address_space(4)));
 // per-item address arithmetic out of the loop: 191 -> 256 VGPRs plus AGPR
 // spills at N = 3, half the occupancy.)
 template <int N, int WT, int STAGE, bool NT>
-__global__ __launch_bounds__(kBlock) void ensemble_stage_kernel(const StageParams* __restrict__ params,
+__global__ __launch_bounds__(group_threads(WT)) void ensemble_stage_kernel(const StageParams* __restrict__ params,
                                                                 const int2* __restrict__ items) {
   __shared__ double lds[stage_lds_doubles(N, kMaxwellTM, WT)];
   ConstParams* base = reinterpret_cast<ConstParams*>(reinterpret_cast<uintptr_t>(params));
@@ -73,7 +74,19 @@ EnsFn stage_of(int stage) {
 
 template <bool NT>
 EnsFn ensemble_fn(int order, int wt, int stage) {
-  if (wt != kMfmaTiling && wt != 1 && wt != 2 && wt != 4) return nullptr;  // 256-thread workgroups only
+  if (wt == 8) {  // one tile per 512-thread workgroup
+    switch (order) {
+      case 1: return stage_of<1, 8, NT>(stage);
+      case 2: return stage_of<2, 8, NT>(stage);
+      case 3: return stage_of<3, 8, NT>(stage);
+      case 4: return stage_of<4, 8, NT>(stage);
+      case 5: return stage_of<5, 8, NT>(stage);
+      case 6: return stage_of<6, 8, NT>(stage);
+      case 7: return stage_of<7, 8, NT>(stage);
+      default: return nullptr;
+    }
+  }
+  if (wt != kMfmaTiling && wt != 1 && wt != 2 && wt != 4) return nullptr;
   switch (order * 8 + wt) {
     case 1 * 8 + 2: return stage_of<1, 2, NT>(stage);
     case 1 * 8 + 4: return stage_of<1, 4, NT>(stage);

@@ -30,6 +30,21 @@
 
 namespace dgc {
 
+// Code shape of the volume and lift loops, measured per operator, order and
+// tiling (whole RK4 steps at K = 8.4M for N <= 3 and K = 131072 above).
+// Unrolled completely, the loops let the compiler hoist every LDS read and
+// coefficient load, which for three fields costs 190-256 VGPRs (1-2 waves per
+// SIMD); unrolled by 4 with the stage operands loaded after the products, the
+// Maxwell kernels hold 66-110 VGPRs (Maxwell N = 6 at 8 waves per tile: 907
+// -> 4282 GB/s; N = 3 at 4 waves: 3601 -> 4495 GB/s). The single-field
+// advection kernels keep the complete unroll, except at 8 waves per tile for
+// N >= 6 (1319 -> 3568 GB/s at N = 6).
+constexpr int stage_unroll(int N, int PHYS, int WT) {
+  if (PHYS == kMaxwellTM) return (N == 2 && WT <= 2) ? 64 : 4;
+  return (WT == 8 && N >= 6) ? 2 : 64;
+}
+constexpr bool stage_delays_operands(int N, int PHYS, int WT) { return stage_unroll(N, PHYS, WT) < 64; }
+
 // One workgroup's share of a stage: the TPW = 4/WT tiles of tile group
 // `group` (cells group*64*TPW ...), with `lds` holding stage_lds_doubles()
 // doubles. Used by the per-order kernels below and by the ensemble kernel.
@@ -44,6 +59,8 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   constexpr int LPW = (Np + WT - 1) / WT;  // input rows loaded per wave (strided)
   constexpr int FPW = (NF + WT - 1) / WT;  // face slots per wave (strided)
   constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
+  constexpr bool kDelayOperands = stage_delays_operands(N, PHYS, WT);
+  constexpr int kUnroll = stage_unroll(N, PHYS, WT);
   using UsT = double[F][Np][kTile];
   using FlT = double[F][NF][kTile];
   UsT* us = reinterpret_cast<UsT*>(lds);                        // us[TPW][F][Np][kTile]
@@ -165,10 +182,12 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   }
   __syncthreads();
 
-  // ---- 3. stage operands of this wave's rows (in flight during the FMAs) --
+  // ---- 3. stage operands of this wave's rows (in flight during the FMAs, or
+  //         loaded after the products where the code shape says so) ---------
   const int j0 = wt * RPW;
   if (j0 >= Np) return;  // a wave without output rows (no barrier follows)
   double o1[F][RPW], o2[F][RPW], o3[F][RPW];
+  auto load_operands = [&]() {
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
     const int j = j0 + i;
@@ -192,13 +211,16 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
     }
   }
 
+  };
+  if constexpr (!kDelayOperands) load_operands();
+
   // ---- 4. volume term: Dr.u and Ds.u for this wave's rows -------------------
   double rr[F][RPW], ss[F][RPW];
 #pragma unroll
   for (int f = 0; f < F; ++f)
 #pragma unroll
     for (int i = 0; i < RPW; ++i) rr[f][i] = ss[f][i] = 0.0;
-#pragma unroll
+#pragma unroll kUnroll
   for (int k = 0; k < Np; ++k) {
     double uk[F];
 #pragma unroll
@@ -231,7 +253,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   }
 
   // ---- 5. lift: res -= LIFT . (flux * face_J / J) ----------------------------
-#pragma unroll
+#pragma unroll kUnroll
   for (int m = 0; m < NF; ++m) {
     double fm[F];
 #pragma unroll
@@ -245,6 +267,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
     }
   }
 
+  if constexpr (kDelayOperands) load_operands();
   // ---- 6. stage update (reference operand order: out = a * x + y) ------------
   if (!valid) return;
 #pragma unroll

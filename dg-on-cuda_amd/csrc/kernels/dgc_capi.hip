@@ -780,10 +780,9 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
     }
   }
   hipStream_t st = op->disc->ctx->stream;
-  // 8 and 16 waves per tile (one tile per workgroup) only pay on meshes of a
-  // few tiles; their times are not reported in ms_out
-  const int tiles = (op->disc->K + dgc::kTile - 1) / dgc::kTile;
-  const int ncand = tiles <= 256 ? 6 : 4;
+  // 8 and 16 waves per tile (one tile per 512- or 1024-thread workgroup):
+  // timed like the others, not reported in ms_out
+  const int ncand = 6;
   const int cand[6] = {1, 2, 4, dgc::kMfmaTiling, 8, 16};
   float best = 0.f;
   int best_wt = op->wt;
@@ -974,7 +973,7 @@ static int ensemble_build(dgc_ensemble_s* e) {
     }
     EnsembleGroup& g = e->groups.back();
     e->wt[i] = g.wt;
-    const int cells = g.wt == dgc::kMfmaTiling ? dgc::kTile : dgc::kTile * (dgc::kWaves / g.wt);
+    const int cells = g.wt == dgc::kMfmaTiling ? dgc::kTile : dgc::kTile * dgc::tiles_per_group(g.wt);
     const int ngroups = (e->ops[i]->disc->K + cells - 1) / cells;
     for (int t = 0; t < ngroups; ++t) items.push_back(make_int2(i, t));
     g.nitems = int(items.size()) - g.offset;

@@ -241,8 +241,7 @@ int dgc_op_num_fields(dgc_op_t op);
  * matrix-core variant (Dr u, Ds u and LIFT flux as 16x16x4 f64 MFMA
  * products, one 64-cell tile per workgroup; FP64 throughout). A tiling whose
  * LDS would not fit falls back to 4. dgc_op_autotune times an RK4 step (the 4
- * stage kernels) of every built tiling (8 and 16 only on meshes of at most 256
- * tiles) on the caller's buffers (work vectors only are written: stage 4
+ * stage kernels) of every built tiling on the caller's buffers (work vectors only are written: stage 4
  * updates wk2 in place of the state) and keeps the fastest; ms_out[4]
  * receives the mean stage-2 kernel time of wt = 1, 2, 4, 0 (-1 where not
  * built). No reference counterpart. */
