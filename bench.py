@@ -375,6 +375,35 @@ def ipdg_bench(dgc, ctx, n_side, N, gx, n_train, epochs, n_eval, n_eval_bj, mixe
     return out
 
 
+def solver_trace(dgc, ctx, n_side=128, N=3):
+    """The IPDG solvers of the learned-preconditioner comparison (K = 32768,
+    N = 3) with every iteration's kernels launched one by one instead of from
+    captured graphs, for kernel traces: set-up, a short training (16 solves x 16
+    iterates, one epoch), then one solve each of block-Jacobi PCG, two-level
+    PCG and flexible CG with the network."""
+    mesh = dgc.Mesh.structured(n_side, n_side, -1, 1, -1, 1, jitter=0.1, seed=3)
+    disc = dgc.Discretization(ctx, mesh, N)
+    op = dgc.IPDG(disc, tau_mode=1)
+    op.set_graphs(False)
+    op.block_jacobi()
+    op.coarse_space(mesh, 32)
+    net = dgc.LearnedPreconditioner(op, seed=0, base=2)
+    pairs = net.build_dataset_solver(mesh, 16, seed0=100, modes=16, snapshots=16)
+    net.train(0, pairs, epochs=1, lr=3e-4, seed=0)
+    xg, yg = mesh.dof_coords(N, node_major=True)
+    b = dgc.DeviceArray(ctx, disc.size)
+    x = dgc.DeviceArray(ctx, disc.size)
+    f, _ = dgc.poisson_rhs(xg, yg, seed=10000)
+    op.mass(dgc.DeviceArray.from_host(ctx, f), b)
+    out = dict(K=disc.K, N=N, pairs=pairs)
+    for name, fn in (("bj_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=1)),
+                     ("two_level_pcg", lambda: op.pcg(b, x, rtol=1e-10, precond=2)),
+                     ("learned_fcg", lambda: net.fcg(b, x, rtol=1e-10, stall_window=40, stall_factor=0.5))):
+        r = fn()
+        out[name] = dict(iters=r["iters"], ms=round(r["ms"], 3), fell_back=r.get("fell_back", 0))
+    return out
+
+
 def ipdg_apply_bytes(K, N):
     """Algorithmic bytes of one SPD apply (pass 1: read u, the 4 metric terms
     and 3 connectivity ints per cell (normals and face_J / J follow from the
@@ -722,6 +751,10 @@ def main():
                          "kernel-traced)")
     ap.add_argument("--widened-only", action="store_true",
                     help="run only the gradient / divergence, Stokes and 1D advection timings (for kernel traces)")
+    ap.add_argument("--solver-trace", action="store_true",
+                    help="run only the IPDG solvers (K = 32768, N = 3) with uncaptured launches (for kernel traces)")
+    ap.add_argument("--maxwell-order", type=int, default=0,
+                    help="run only Maxwell TM at this order on the 256 x 256 mesh (for counter collection)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ipdg-quick", action="store_true",
                     help="IPDG comparison on a small sample (512 training pairs, 8 held-out right-hand sides) "
@@ -752,6 +785,16 @@ def main():
         if not args.no_stokes:
             out["stokes"] = stokes_bench(dgc, ctx)
         print(json.dumps(out))
+        return
+    if args.solver_trace:
+        print(json.dumps(dict(workload="ipdg solvers K=32768 N=3, uncaptured", **solver_trace(dgc, ctx))))
+        return
+    if args.maxwell_order:
+        mesh = dgc.Mesh.structured(256, 256, 0, 1, 0, 1, jitter=0.05, seed=1, periodic=True)
+        r = run_order(dgc, ctx, mesh, args.maxwell_order, args.steps, args.warmup, physics="maxwell")
+        print(json.dumps(dict(workload=f"maxwell fp64 rk4 K={r['K']} N={args.maxwell_order}", value=r["value"],
+                              waves_per_tile=r["tune"]["choice"], roofline=roofline_for(r),
+                              stage_us=[round(v * 1e3, 2) for v in r["stage_ms"]])))
         return
     if args.ipdg_apply_only:
         print(json.dumps(dict(workload="ipdg spd apply N=3", ipdg_apply=ipdg_apply_bench(dgc, ctx))))

@@ -108,15 +108,12 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, 
     const int code = valid ? net.nbr[(size_t)e * K + c] : kBoundaryCode;
     nb[e] = (code & 3) != kBoundaryCode ? (code >> 2) : -1;
   }
-  floatx16 acc0 = {}, acc1 = {};
+  // every global read of the wave first (own rows, then the neighbour rows
+  // and their bf16-rounded sums), the products after: one round trip
+  uint64_t own[D / 8], nsum[D / 8];
 #pragma unroll
-  for (int ks = 0; ks < D / 8; ++ks) {
-    uint64_t u = 0;
-    if (valid) u = *reinterpret_cast<const uint64_t*>(hin + (size_t)c * D + ks * 8 + 4 * kq);
-    const bf16x4 a = as_bf16x4(u);
-    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 0) * 64 + lane]), acc0);
-    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 1) * 64 + lane]), acc1);
-  }
+  for (int ks = 0; ks < D / 8; ++ks)
+    own[ks] = valid ? *reinterpret_cast<const uint64_t*>(hin + (size_t)c * D + ks * 8 + 4 * kq) : 0;
 #pragma unroll
   for (int ks = 0; ks < D / 8; ++ks) {
     float s[4] = {0.f, 0.f, 0.f, 0.f};
@@ -130,7 +127,18 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, 
     }
     uint64_t u = 0;
     for (int v = 0; v < 4; ++v) u |= uint64_t(f2bf(s[v])) << (16 * v);
-    const bf16x4 a = as_bf16x4(u);
+    nsum[ks] = u;
+  }
+  floatx16 acc0 = {}, acc1 = {};
+#pragma unroll
+  for (int ks = 0; ks < D / 8; ++ks) {
+    const bf16x4 a = as_bf16x4(own[ks]);
+    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 0) * 64 + lane]), acc0);
+    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 1) * 64 + lane]), acc1);
+  }
+#pragma unroll
+  for (int ks = 0; ks < D / 8; ++ks) {
+    const bf16x4 a = as_bf16x4(nsum[ks]);
     const int kk = D / 8 + ks;
     acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 0) * 64 + lane]), acc0);
     acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 1) * 64 + lane]), acc1);
@@ -309,37 +317,66 @@ __global__ __launch_bounds__(256) void energy_grad_kernel(MpnnNet net, const dou
 
 // ---- dense layer backward ---------------------------------------------------
 // dz = dh * (h > 0) (in place in dh); u = A^T dz, v = B^T dz for the layer below.
+// A workgroup owns 64 cells (lane = cell); its four waves split the D outputs
+// and read the layer's weights, widened to fp32, from LDS (transposed, so a
+// lane's 64 products of one output stream through 16-byte broadcast reads).
+// Each u, v entry is the same fp32 multiply-add chain over o = 0..63 as the
+// oracle's restatement.
 template <int D>
 __global__ __launch_bounds__(256) void dense_bwd_kernel(MpnnNet net, int layer, const uint16_t* __restrict__ hl,
                                                         float* __restrict__ dh, float* __restrict__ u,
                                                         float* __restrict__ v) {
+  __shared__ float4 at[D][kHid / 4], bt[D][kHid / 4];  // A^T, B^T: [input i][output o]
   const int K = net.K;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= K) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const bool valid = c < K;
+  if (u) {
+    ConstU32* A = weight_pairs(net.wbf + net.L.oA[layer]);
+    ConstU32* B = weight_pairs(net.wbf + net.L.oB[layer]);
+    float* af = reinterpret_cast<float*>(at);
+    float* bfl = reinterpret_cast<float*>(bt);
+    for (int idx = threadIdx.x; idx < D * kHid; idx += 256) {
+      const int i = idx / kHid, o = idx - i * kHid;
+      af[idx] = weight_at(A, o * D + i);
+      bfl[idx] = weight_at(B, o * D + i);
+    }
+  }
   float dz[kHid];
 #pragma unroll
   for (int q4 = 0; q4 < kHid / 4; ++q4) {
-    float t[4];
-    load_bf16x4(hl + (size_t)c * kHid + 4 * q4, t);
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) load_bf16x4(hl + (size_t)c * kHid + 4 * q4, t);
     for (int w = 0; w < 4; ++w) {
       const int o = 4 * q4 + w;
-      const float g = dh[(size_t)c * kHid + o];
+      const float g = valid ? dh[(size_t)c * kHid + o] : 0.f;
       dz[o] = t[w] > 0.f ? g : 0.f;
-      dh[(size_t)c * kHid + o] = dz[o];
     }
   }
-  if (!u) return;
-  ConstU32* A = weight_pairs(net.wbf + net.L.oA[layer]);
-  ConstU32* B = weight_pairs(net.wbf + net.L.oB[layer]);
-  for (int i = 0; i < D; ++i) {
+  __syncthreads();  // every wave has read dh (and the weights are staged)
+  if (valid && wave == 0) {
+#pragma unroll
+    for (int o = 0; o < kHid; ++o) dh[(size_t)c * kHid + o] = dz[o];
+  }
+  if (!u) return;  // u is a kernel argument: the whole grid returns here or none of it
+  for (int i = wave; i < D; i += 4) {
     float su = 0.f, sv = 0.f;
 #pragma unroll
-    for (int o = 0; o < kHid; ++o) {
-      su = fmaf(weight_at(A, o * D + i), dz[o], su);
-      sv = fmaf(weight_at(B, o * D + i), dz[o], sv);
+    for (int q = 0; q < kHid / 4; ++q) {
+      const float4 a4 = at[i][q], b4 = bt[i][q];
+      su = fmaf(a4.x, dz[4 * q + 0], su);
+      sv = fmaf(b4.x, dz[4 * q + 0], sv);
+      su = fmaf(a4.y, dz[4 * q + 1], su);
+      sv = fmaf(b4.y, dz[4 * q + 1], sv);
+      su = fmaf(a4.z, dz[4 * q + 2], su);
+      sv = fmaf(b4.z, dz[4 * q + 2], sv);
+      su = fmaf(a4.w, dz[4 * q + 3], su);
+      sv = fmaf(b4.w, dz[4 * q + 3], sv);
     }
-    u[(size_t)c * D + i] = su;
-    v[(size_t)c * D + i] = sv;
+    if (valid) {
+      u[(size_t)c * D + i] = su;
+      v[(size_t)c * D + i] = sv;
+    }
   }
 }
 
@@ -532,7 +569,7 @@ hipError_t mpnn_dense_bwd(const MpnnNet& net, int layer, const uint16_t* hl, flo
                           hipStream_t s) {
   // layer is the index of h^{layer+1}'s weights; u, v have the width of its input
   const int D = net.L.din(layer);
-  const dim3 grid((net.K + 255) / 256);
+  const dim3 grid((net.K + 63) / 64);
   switch (D) {
     case 16: hipLaunchKernelGGL(dense_bwd_kernel<16>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;
     case 24: hipLaunchKernelGGL(dense_bwd_kernel<24>, grid, dim3(256), 0, s, net, layer, hl, dh, u, v); break;

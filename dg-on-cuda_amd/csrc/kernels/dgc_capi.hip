@@ -760,14 +760,11 @@ int dgc_op_cache_streaming(dgc_op_t op) { return op ? op_streaming(op) : -1; }
 // written, which the next step's stage 1 re-initialises, so the state is left
 // untouched. ms_out[i] receives the mean stage-2 kernel time of candidate i
 // (timed on its own).
-int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
-                    float* ms_out /* [4]: wt = 1, 2, 4, 0; -1 if not built */) {
-  int rc = check_step_args(op, inout, size, wk, "dgc_op_autotune");
-  if (rc != DGC_OK) return rc;
-  DGC_REQUIRE(reps > 0, "dgc_op_autotune: reps must be positive");
+// Stage parameters of the timing entry points: stage s reads and writes what
+// it does in one RK4 step, except that stage 4 updates wk2 (free once stage 3
+// has read it) in place of the state; with c1 = 0 only work vectors change.
+static void timing_params(dgc_op_s* op, double* const* inout, double* const* wk, dgc::StageParams p[5]) {
   const int F = op->F;
-  // stage s reads in, writes out (and acc / k3 / u), as in one RK4 step
-  dgc::StageParams p[5];
   for (int s = 1; s <= 4; ++s) {
     p[s] = base_params(op);
     p[s].c1 = 0.0;
@@ -779,6 +776,38 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
       p[s].out[f] = wk[(s == 2 ? 2 : 1) * F + f];
     }
   }
+}
+
+int dgc_op_stage_time(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int stage, int reps,
+                      float* ms) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_op_stage_time");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(stage >= 1 && stage <= 4 && reps > 0 && ms != nullptr, "dgc_op_stage_time: bad stage, reps or ms");
+  dgc::StageParams p[5];
+  timing_params(op, inout, wk, p);
+  hipStream_t st = op->disc->ctx->stream;
+  const int order = op->disc->order, phys = op->physics;
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  hipError_t e = dgc::launch_stage(order, phys, stage, op->wt, p[stage], st);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(ev.start, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = dgc::launch_stage(order, phys, stage, op->wt, p[stage], st);
+  if (e == hipSuccess) e = hipEventRecord(ev.stop, st);
+  if (e == hipSuccess) e = hipEventSynchronize(ev.stop);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ev.start, ev.stop);
+  if (e != hipSuccess) return dgc_set_error(DGC_ERR_LAUNCH, std::string("dgc_op_stage_time: ") + hipGetErrorString(e));
+  *ms = t / reps;
+  return DGC_OK;
+}
+
+int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int reps,
+                    float* ms_out /* [4]: wt = 1, 2, 4, 0; -1 if not built */) {
+  int rc = check_step_args(op, inout, size, wk, "dgc_op_autotune");
+  if (rc != DGC_OK) return rc;
+  DGC_REQUIRE(reps > 0, "dgc_op_autotune: reps must be positive");
+  dgc::StageParams p[5];
+  timing_params(op, inout, wk, p);
   hipStream_t st = op->disc->ctx->stream;
   // 8 and 16 waves per tile (one tile per 512- or 1024-thread workgroup):
   // timed like the others, not reported in ms_out

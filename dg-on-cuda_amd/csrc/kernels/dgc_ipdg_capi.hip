@@ -333,7 +333,8 @@ template <typename T>
 static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, int* iters) {
   hipStream_t st = s->disc->ctx->stream;
   const int chunk = 16;  // even: every chunk starts on direction buffer 0
-  if (!L.graph || L.graph_hist != history || L.graph_P != L.use_P || L.graph_coarse != L.use_coarse) {
+  if (s->use_graphs &&
+      (!L.graph || L.graph_hist != history || L.graph_P != L.use_P || L.graph_coarse != L.use_coarse)) {
     if (L.graph) (void)hipGraphExecDestroy(L.graph);
     L.graph = nullptr;
     hipGraph_t g = nullptr;
@@ -370,7 +371,14 @@ static int cg_run(dgc_ipdg_s* s, IpdgLevel<T>& L, double* history, int maxit, in
       break;
     }
     if (int(h[dgc::kIter]) >= maxit) break;
-    DGC_HIP_TRY(hipGraphLaunch(L.graph, st));
+    if (s->use_graphs) {
+      DGC_HIP_TRY(hipGraphLaunch(L.graph, st));
+    } else {
+      for (int i = 0; i < chunk; ++i) {
+        int rc = cg_iteration(s, L, history, i);
+        if (rc) return rc;
+      }
+    }
     snapped = false;
     if (snapping && s->snap_taken < s->nsnap) {
       DGC_HIP_TRY(hipMemcpyAsync(s->snap + size_t(s->snap_taken) * n, L.x, n * sizeof(double),
@@ -549,6 +557,12 @@ int dgc_ipdg_set_boundary(dgc_ipdg_t s, int kind) {
   DGC_HIP_TRY(hipStreamSynchronize(s->disc->ctx->stream));
   s->neumann = kind;
   operator_changed(s);
+  return DGC_OK;
+}
+
+int dgc_ipdg_set_graphs(dgc_ipdg_t s, int enabled) {
+  DGC_REQUIRE(s != nullptr, "dgc_ipdg_set_graphs: null solver");
+  s->use_graphs = enabled != 0;
   return DGC_OK;
 }
 

@@ -70,6 +70,9 @@ struct dgc_ipdg_s {
   double* snap = nullptr;
   int nsnap = 0;
   int snap_taken = 0;
+  // Solver iterations are replayed from captured graphs (default) or, for
+  // kernel tracing, launched one by one (same kernels, same order).
+  bool use_graphs = true;
 };
 
 // z += Phi A_c^{-1} Phi^T r with partials (r.z, r.r) or (z.z, r.r) into `partial`.

@@ -732,7 +732,7 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     return DGC_OK;
   };
   const int chunk = 8;
-  if (!m->graph || m->graph_hist != s->hist || m->graph_gen != s->coarse_gen) {
+  if (s->use_graphs && (!m->graph || m->graph_hist != s->hist || m->graph_gen != s->coarse_gen)) {
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
     m->graph = nullptr;
     hipGraph_t g = nullptr;
@@ -781,7 +781,12 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
         break;
       }
     }
-    DGC_HIP_TRY(hipGraphLaunch(m->graph, st));
+    if (s->use_graphs) {
+      DGC_HIP_TRY(hipGraphLaunch(m->graph, st));
+    } else {
+      for (int i = 0; i < chunk; ++i)
+        if ((rc = iteration())) return rc;
+    }
   }
   const bool nan_state = !(h[dgc::kRr] == h[dgc::kRr]);
   if (nan_state) {

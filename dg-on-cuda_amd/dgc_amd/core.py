@@ -96,6 +96,7 @@ def _declare(L):
     d("dgc_ipdg_apply_inhomogeneous", [_vp, _vp, _vp, _vp])
     d("dgc_ipdg_apply_data", [_vp, _vp, _vp, _vp])
     d("dgc_ipdg_set_shift", [_vp, C.c_double])
+    d("dgc_ipdg_set_graphs", [_vp, C.c_int])
     d("dgc_ipdg_assemble", [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _ip, _dp])
     d("dgc_ipdg_destroy", [_vp])
     d("dgc_ipdg_apply", [_vp, _vp])
@@ -116,6 +117,8 @@ def _declare(L):
     d("dgc_op_set_cache_policy", [_vp, C.c_int])
     d("dgc_op_cache_streaming", [_vp])
     d("dgc_op_autotune", [_vp, C.POINTER(_vp), C.c_size_t, C.POINTER(_vp), C.c_int, C.POINTER(C.c_float)])
+    d("dgc_op_stage_time", [_vp, C.POINTER(_vp), C.c_size_t, C.POINTER(_vp), C.c_int, C.c_int,
+                            C.POINTER(C.c_float)])
     for name, args in _EXTRA_DECLS:
         d(name, args)
 
@@ -506,7 +509,17 @@ class _Operator:
         ms = (C.c_float * 4)()
         check(lib().dgc_op_autotune(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
                                     reps, ms))
-        return dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4, 0), ms) if m >= 0})
+        out = dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4, 0), ms) if m >= 0})
+        if out["choice"] not in out["ms"]:  # 8 or 16 waves per tile
+            out["ms"][out["choice"]] = self.stage_time(state, work, 2, reps)
+        return out
+
+    def stage_time(self, state, work, stage=2, reps=20) -> float:
+        """Mean milliseconds of back-to-back launches of one stage kernel at the current tiling."""
+        ms = C.c_float()
+        check(lib().dgc_op_stage_time(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
+                                      stage, reps, C.byref(ms)))
+        return ms.value
 
     def profile_stages(self, state, t0, dt, nsteps, work):
         """Mean duration (ms) of each of the 4 stage kernels over nsteps steps."""

@@ -61,6 +61,11 @@ class IPDG:
         check(lib().dgc_ipdg_apply_data(self._h, _vp(gd.ptr) if gd is not None else None,
                                         _vp(hd.ptr) if hd is not None else None, _vp(u.ptr)))
 
+    def set_graphs(self, enabled: bool):
+        """Replay solver iterations from captured graphs (default) or launch the
+        kernels one by one (for kernel tracing); the results are identical."""
+        check(lib().dgc_ipdg_set_graphs(self._h, int(bool(enabled))))
+
     def set_shift(self, sigma: float):
         """SPD form -M Lap_h + sigma M for apply_spd, the block-Jacobi blocks and pcg."""
         check(lib().dgc_ipdg_set_shift(self._h, float(sigma)))

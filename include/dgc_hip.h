@@ -246,6 +246,12 @@ int dgc_op_num_fields(dgc_op_t op);
  * receives the mean stage-2 kernel time of wt = 1, 2, 4, 0 (-1 where not
  * built). No reference counterpart. */
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt);
+/* Mean duration (*ms) of reps back-to-back launches of stage `stage` (1..4) at
+ * the current tiling, HIP events on the context stream, with the buffers of
+ * dgc_op_autotune (work vectors only are written). Measurement entry point;
+ * no reference counterpart. */
+int dgc_op_stage_time(dgc_op_t op, double* const* inout, size_t size, double* const* wk, int stage, int reps,
+                      float* ms);
 int dgc_op_waves_per_tile(dgc_op_t op);
 /* Arithmetic of the stage contractions (no reference counterpart).
  * 0: FP64 throughout (default). 1: Dr u, Ds u and LIFT flux on the matrix
@@ -379,6 +385,11 @@ int dgc_ipdg_apply_data(dgc_ipdg_t s, const double* gd, const double* hd, double
  * then with host arrays row_ptr, col_idx [nnz], values [nnz]. Row block c
  * holds the columns of the cells within two faces of c (sorted), node by node. */
 int dgc_ipdg_assemble(dgc_ipdg_t s, long long* nnz, long long* row_ptr, int* col_idx, double* values);
+/* Launch mode of the solvers' iterations (no reference counterpart): 1
+ * (default) replays captured graphs of 16 (PCG) or 8 (flexible CG)
+ * iterations; 0 launches the same kernels one by one, in the same order, for
+ * kernel tracers that do not follow graph launches. Results are identical. */
+int dgc_ipdg_set_graphs(dgc_ipdg_t s, int enabled);
 /* Builds the block-Jacobi preconditioner: the exact diagonal blocks of
  * -M Lap_h, probed with a distance-2 colouring of the cells, inverted per cell. */
 int dgc_ipdg_block_jacobi(dgc_ipdg_t s, int* num_colors);

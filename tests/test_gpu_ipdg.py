@@ -302,3 +302,25 @@ def test_shifted_spd_form_vs_oracle(ctx, N):
     op.set_boundary(0)
     op.apply_spd(u, out)
     assert np.array_equal(out.download(), plain)
+
+
+def test_uncaptured_solver_launches_give_identical_solves(ctx):
+    """dgc_ipdg_set_graphs(0) launches the iterations' kernels one by one (for
+    kernel tracers) instead of replaying captured graphs: the same iterates,
+    bit for bit, for block-Jacobi and two-level PCG."""
+    N = 3
+    mesh = dgc_amd.Mesh.structured(16, 16, -1, 1, -1, 1, jitter=0.1, seed=5)
+    disc = dgc_amd.Discretization(ctx, mesh, N)
+    op = dgc_amd.IPDG(disc, tau_mode=1)
+    op.coarse_space(mesh, 4)
+    b = dgc_amd.DeviceArray.from_host(ctx, np.random.default_rng(7).standard_normal(disc.size))
+    for precond in (1, 2):
+        xs = []
+        for graphs in (True, False):
+            op.set_graphs(graphs)
+            x = dgc_amd.DeviceArray(ctx, disc.size)
+            r = op.pcg(b, x, rtol=1e-10, maxit=5000, precond=precond)
+            xs.append((r["iters"], x.download()))
+        op.set_graphs(True)
+        assert xs[0][0] == xs[1][0]
+        np.testing.assert_array_equal(xs[0][1], xs[1][1])
