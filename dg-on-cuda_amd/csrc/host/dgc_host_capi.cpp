@@ -5,6 +5,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../../include/dgc_hip.h"
 #include "mesh.hpp"
@@ -17,6 +18,13 @@ struct dgc_mesh_s {
 };
 
 namespace {
+// Copies a host vector to a caller array (nothing for an empty vector, whose
+// data() may be null: memcpy from a null pointer is undefined even for 0 bytes).
+template <class T>
+void copy_out(T* dst, const std::vector<T>& v, size_t count) {
+  if (dst != nullptr && count > 0) std::memcpy(dst, v.data(), count * sizeof(T));
+}
+
 template <class Fn>
 int guarded(const char* what, Fn&& fn) {
   try {
@@ -135,16 +143,16 @@ int dgc_mesh_geometry(dgc_mesh_t mesh, int order, double* inv_jac, double* J, do
     dgc_host::triangle_nodes(order, r, s);
     const dgc_host::Geometry g = dgc_host::build_geometry(mesh->mesh, order, r, s);
     const size_t K = size_t(g.K);
-    if (inv_jac) std::memcpy(inv_jac, g.inv_jac.data(), 4 * K * sizeof(double));
-    if (J) std::memcpy(J, g.J.data(), K * sizeof(double));
-    if (face_J) std::memcpy(face_J, g.face_J.data(), 3 * K * sizeof(double));
-    if (nx) std::memcpy(nx, g.nx.data(), 3 * K * sizeof(double));
-    if (ny) std::memcpy(ny, g.ny.data(), 3 * K * sizeof(double));
-    if (iface_cell) std::memcpy(iface_cell, g.iface_cell.data(), 3 * K * sizeof(int));
-    if (iface_face) std::memcpy(iface_face, g.iface_face.data(), 3 * K * sizeof(int));
+    copy_out(inv_jac, g.inv_jac, 4 * K);
+    copy_out(J, g.J, K);
+    copy_out(face_J, g.face_J, 3 * K);
+    copy_out(nx, g.nx, 3 * K);
+    copy_out(ny, g.ny, 3 * K);
+    copy_out(iface_cell, g.iface_cell, 3 * K);
+    copy_out(iface_face, g.iface_face, 3 * K);
     if (num_boundary_nodes) *num_boundary_nodes = g.num_boundary_nodes;
-    if (bx) std::memcpy(bx, g.bnode_x.data(), g.bnode_x.size() * sizeof(double));
-    if (by) std::memcpy(by, g.bnode_y.data(), g.bnode_y.size() * sizeof(double));
+    copy_out(bx, g.bnode_x, g.bnode_x.size());
+    copy_out(by, g.bnode_y, g.bnode_y.size());
     return DGC_OK;
   });
 }
