@@ -118,8 +118,17 @@ def mfma_stats(r, peak=MFMA_PEAK_F32_TFLOPS):
 # calibration factors x2.000 and x1.000), keyed by (physics, N, K, waves per tile)
 # of the stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
 # profiles/dg_stage_kernel_N3_K8388608_hbm.md hold the runs.
-MEASURED_TRAFFIC = {("advection", 7, 131072, 4): 196.8e6, ("advection", 3, 8388608, 2): 3737.7e6,
-                    ("maxwell", 3, 8388608, 2): 10754.0e6}
+# HBM traffic per stage-2 launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one run each, scaled x2.000 and x1.000 by the same counters on
+# dgc_axpy of known size): profiles/dg_stage_kernel_advection_N7_K131072_pmc.md
+# and profiles/dg_stage_kernel_N3_K8388608_hbm.md. Keyed by (physics, N, K,
+# waves per tile; 0 = FP64 matrix-core variant); reported as roofline.traffic
+# with its source, not measured inside the bench run.
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/ (not measured in this run)"
+MEASURED_TRAFFIC = {("advection", 7, 131072, 16): 196.5e6, ("advection", 7, 131072, 8): 196.8e6,
+                    ("advection", 7, 131072, 4): 196.8e6, ("advection", 7, 131072, 2): 197.1e6,
+                    ("advection", 7, 131072, 0): 197.2e6,
+                    ("advection", 3, 8388608, 2): 3737.7e6, ("maxwell", 3, 8388608, 2): 10754.0e6}
 
 
 def roofline_for(r):
@@ -136,7 +145,8 @@ def roofline_for(r):
     name = (f"dg_stage_mfma_kernel<double, N={r['N']}, {phys}, stage 2{hint}>" if wt == 0 else
             f"dg_stage_kernel<N={r['N']}, {phys}, stage 2, {wt} waves per tile{hint}>")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_source=TRAFFIC_SOURCE if traffic else None, kernel=name,
                 bytes_per_launch=int(b), avg_launch_us=round(dur * 1e6, 2))
 
 

@@ -429,6 +429,7 @@ static void release_coarse(dgc_ipdg_s* s);
 // boundary kind or shift: block inverses, coarse space, captured solver
 // graphs (they hold the old parameters).
 static void operator_changed(dgc_ipdg_s* s) {
+  ++s->op_gen;
   release_coarse(s);
   (void)hipFree(s->f64.P);
   s->f64.P = nullptr;

@@ -63,6 +63,7 @@ struct dgc_ipdg_s {
   std::vector<void*> coarse_owned;
   std::vector<double> coarse_A;  // Phi^T S Phi (host copy, row-major)
   int coarse_gen = 0;            // bumped whenever arrays that solver graphs capture change
+  int op_gen = 0;                // bumped whenever the SPD operator changes (boundary kind, shift)
   // Iterates recorded by the FP64 PCG for the learned preconditioner's training
   // data: with snap set, the first run of a solve copies x after every
   // graph-replayed chunk of iterations into snap[taken * n] while taken < nsnap

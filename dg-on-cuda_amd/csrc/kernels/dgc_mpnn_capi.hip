@@ -67,6 +67,7 @@ struct dgc_mpnn_s {
   double* dotpart = nullptr;      // [256] + 1 result slot
   int adam_step = 0;
   int base = 1;  // 1: block-Jacobi, 2: two-level (block-Jacobi + coarse correction)
+  int op_gen = 0;  // the solver's operator generation the network was built for
   double ct = 1.0;
   hipGraphExec_t graph = nullptr;
   double* graph_hist = nullptr;
@@ -289,6 +290,16 @@ int set_target_scale_from(dgc_mpnn_s* m, const double* r) {
   return DGC_OK;
 }
 
+// The network, its training data and target scale belong to the operator it
+// was created for; a later change of the boundary kind or shift makes them stale.
+int check_fresh(const dgc_mpnn_s* m, const char* what) {
+  if (m->op_gen != m->s->op_gen || m->s->f64.P == nullptr)
+    return dgc_set_error(DGC_ERR_INVALID, std::string(what) +
+                                              ": the operator changed (boundary kind or shift) since the network "
+                                              "was created; create and train a new network");
+  return DGC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -304,6 +315,7 @@ int dgc_mpnn_create(dgc_ipdg_t s, uint64_t seed, int base, dgc_mpnn_t* out) {
   auto* m = new dgc_mpnn_s;
   m->s = s;
   m->base = base;
+  m->op_gen = s->op_gen;
   m->K = s->disc->K;
   m->N = s->disc->order;
   m->Np = s->disc->Np;
@@ -452,6 +464,7 @@ int dgc_mpnn_set_target_scale(dgc_mpnn_t m, double ct) {
 
 int dgc_mpnn_energy_loss_grad(dgc_mpnn_t m, const double* r, double er, double* loss, float* grad, double* z) {
   DGC_REQUIRE(m && r, "dgc_mpnn_energy_loss_grad: null argument");
+  if (int rc_ = check_fresh(m, "dgc_mpnn_energy_loss_grad")) return rc_;
   DGC_REQUIRE(std::isfinite(er) && er > 0, "dgc_mpnn_energy_loss_grad: need e.r > 0");
   hipStream_t st = stream_of(m);
   int rc = energy_step(m, r, er, grad != nullptr, loss);
@@ -466,6 +479,7 @@ int dgc_mpnn_energy_loss_grad(dgc_mpnn_t m, const double* r, double er, double* 
 int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nsamples, uint64_t seed0,
                            int modes) {
   DGC_REQUIRE(m && dof_x && dof_y, "dgc_mpnn_build_dataset: null argument");
+  if (int rc_ = check_fresh(m, "dgc_mpnn_build_dataset")) return rc_;
   DGC_REQUIRE(nsamples > 0 && modes >= 1 && modes <= 32, "dgc_mpnn_build_dataset: bad nsamples or modes");
   hipStream_t st = stream_of(m);
   const size_t n = size_t(m->K) * m->Np;
@@ -520,6 +534,7 @@ int dgc_mpnn_build_dataset(dgc_mpnn_t m, const double* dof_x, const double* dof_
 int dgc_mpnn_build_dataset_solver(dgc_mpnn_t m, const double* dof_x, const double* dof_y, int nrhs,
                                   uint64_t seed0, int modes, int snapshots, int* npairs) {
   DGC_REQUIRE(m && dof_x && dof_y, "dgc_mpnn_build_dataset_solver: null argument");
+  if (int rc_ = check_fresh(m, "dgc_mpnn_build_dataset_solver")) return rc_;
   DGC_REQUIRE(nrhs > 0 && snapshots > 0 && modes >= 1 && modes <= 32,
               "dgc_mpnn_build_dataset_solver: bad nrhs, snapshots or modes");
   dgc_ipdg_s* s = m->s;
@@ -599,6 +614,7 @@ int dgc_mpnn_build_dataset_solver(dgc_mpnn_t m, const double* dof_x, const doubl
 int dgc_mpnn_train(dgc_mpnn_t m, int first, int count, int epochs, float lr, uint64_t shuffle_seed,
                    double* epoch_loss) {
   DGC_REQUIRE(m, "dgc_mpnn_train: null network");
+  if (int rc_ = check_fresh(m, "dgc_mpnn_train")) return rc_;
   DGC_REQUIRE(m->nsamples > 0, "dgc_mpnn_train: build a dataset first");
   DGC_REQUIRE(first >= 0 && count > 0 && first + count <= m->nsamples, "dgc_mpnn_train: sample range");
   DGC_REQUIRE(epochs > 0 && lr > 0.f, "dgc_mpnn_train: bad epochs or learning rate");
@@ -633,6 +649,7 @@ int dgc_mpnn_train(dgc_mpnn_t m, int first, int count, int epochs, float lr, uin
 
 int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss) {
   DGC_REQUIRE(m && loss, "dgc_mpnn_eval_loss: null argument");
+  if (int rc_ = check_fresh(m, "dgc_mpnn_eval_loss")) return rc_;
   DGC_REQUIRE(first >= 0 && count > 0 && first + count <= m->nsamples, "dgc_mpnn_eval_loss: sample range");
   const size_t n = size_t(m->K) * m->Np;
   double acc = 0.0;
@@ -656,6 +673,9 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   DGC_REQUIRE(rtol > 0 && maxit > 0, "dgc_ipdg_fcg: need rtol > 0 and maxit > 0");
   DGC_REQUIRE(m->ct > 0, "dgc_ipdg_fcg: the network has no target scale (build a dataset first)");
   DGC_REQUIRE(m->base == 1 || s->coarse.nc > 0, "dgc_ipdg_fcg: the coarse space was released");
+  DGC_REQUIRE(m->op_gen == s->op_gen && s->f64.P != nullptr,
+              "dgc_ipdg_fcg: the operator changed (boundary kind or shift) since the network was created; "
+              "create and train a new network");
   dgc_disc_s* d = s->disc;
   hipStream_t st = d->ctx->stream;
   const size_t n = size_t(d->K) * d->Np;
@@ -681,10 +701,9 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
   DGC_REQUIRE(std::isfinite(bb) && bb > 0, "dgc_ipdg_fcg: zero or non-finite right-hand side");
   const double nd = double(n);
 
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  DGC_HIP_TRY(hipEventRecord(e0, st));
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  DGC_HIP_TRY(hipEventRecord(ev.start, st));
   // ---- initialisation: r = b, x = 0, z = M(r), p = z
   double h[dgc::kCgScalars] = {0, 0, 0, 0, 0, 0, rtol * rtol * bb, double(maxit), 0, 0};
   DGC_HIP_TRY(hipMemcpyAsync(Lv.scal, h, sizeof(h), hipMemcpyHostToDevice, st));
@@ -815,12 +834,10 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
     if ((rc = true_rel(rel))) return rc;
     more = !(rel <= rtol);
   }
-  DGC_HIP_TRY(hipEventRecord(e1, st));
-  DGC_HIP_TRY(hipEventSynchronize(e1));
+  DGC_HIP_TRY(hipEventRecord(ev.stop, st));
+  DGC_HIP_TRY(hipEventSynchronize(ev.stop));
   float el = 0.f;
-  (void)hipEventElapsedTime(&el, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&el, ev.start, ev.stop);
   DGC_HIP_TRY(hipMemcpyAsync(x, Lv.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (history) DGC_HIP_TRY(hipMemcpyAsync(history, s->hist, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
   DGC_HIP_TRY(hipStreamSynchronize(st));

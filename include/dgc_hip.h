@@ -482,7 +482,10 @@ int dgc_mpnn_eval_loss(dgc_mpnn_t m, int first, int count, double* loss);
  * stall_factor^2 times its value stall_window iterations earlier (or turns
  * non-finite), the solve continues with PCG on the network's base
  * preconditioner (block-Jacobi or two-level) from the current iterate
- * (*fell_back = 1; 2 = finished by a PCG restart after drift). */
+ * (*fell_back = 1; 2 = finished by a PCG restart after drift). A network
+ * belongs to the operator it was created for: after dgc_ipdg_set_boundary or
+ * dgc_ipdg_set_shift this call and the training / evaluation entry points
+ * return DGC_ERR_INVALID (create and train a new network). */
 int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double rtol, int maxit,
                  int stall_window, double stall_factor, int* iters, double* relres, int* fell_back,
                  double* history, float* ms);

@@ -185,3 +185,24 @@ def test_training_pairs_from_the_solver_itself(ctx):
     assert 0 < first < late < 1.0
     hist = net.train(0, 12, epochs=2, lr=3e-4, seed=0)
     assert np.all(np.isfinite(hist))
+
+
+def test_network_is_refused_after_an_operator_change(ctx):
+    """The network's training data and target scale belong to the operator it
+    was created for: after dgc_ipdg_set_shift (or set_boundary) flexible CG,
+    training and evaluation return an error instead of running on a stale
+    preconditioner."""
+    mesh, disc, op, b = _poisson(ctx, 8, 2)
+    net = dgc_amd.LearnedPreconditioner(op, seed=0, base=1)
+    net.build_dataset(mesh, 4, seed0=100, modes=4)
+    x = dgc_amd.DeviceArray(ctx, disc.size)
+    assert net.fcg(b, x, rtol=1e-8)["relres"] <= 1e-8
+    op.set_shift(2.0)
+    with pytest.raises(dgc_amd.DGCError, match="operator changed"):
+        net.fcg(b, x, rtol=1e-8)
+    with pytest.raises(dgc_amd.DGCError, match="operator changed"):
+        net.train(0, 4, epochs=1)
+    op.set_shift(0.0)
+    op.pcg(b, x, rtol=1e-8, precond=1)  # rebuilds the block inverses; the network stays stale
+    with pytest.raises(dgc_amd.DGCError, match="operator changed"):
+        net.eval_loss(0, 4)
