@@ -38,9 +38,13 @@ namespace dgc {
 // Maxwell kernels hold 66-110 VGPRs (Maxwell N = 6 at 8 waves per tile: 907
 // -> 4282 GB/s; N = 3 at 4 waves: 3601 -> 4495 GB/s). The single-field
 // advection kernels keep the complete unroll, except at 8 waves per tile for
-// N >= 6 (1319 -> 3568 GB/s at N = 6).
+// N >= 6 (1319 -> 3568 GB/s at N = 6). At 16 waves per tile and N <= 4 a wave
+// owns one output row, so the complete unroll costs few registers and issues
+// every coefficient load in one batch, which on a mesh of a few tiles (where
+// each launch is a chain of dependent latencies) is what counts: the Maxwell
+// cavity of 8 tiles at N = 4, 491 steps, 14.06 -> 13.02 ms.
 constexpr int stage_unroll(int N, int PHYS, int WT) {
-  if (PHYS == kMaxwellTM) return (N == 2 && WT <= 2) ? 64 : 4;
+  if (PHYS == kMaxwellTM) return ((N == 2 && WT <= 2) || (WT == 16 && N <= 4)) ? 64 : 4;
   return (WT == 8 && N >= 6) ? 2 : 64;
 }
 constexpr bool stage_delays_operands(int N, int PHYS, int WT) { return stage_unroll(N, PHYS, WT) < 64; }

@@ -47,6 +47,24 @@ int dgc_set_error(int code, const std::string& msg) {
     if (!(cond)) return dgc_set_error(DGC_ERR_INVALID, msg); \
   } while (0)
 
+// Full RK4 steps the time loops replay from one graph launch. On a mesh of a
+// few tiles a step's four kernels run for a few microseconds each, and every
+// graph launch adds its own dispatch and completion-signal round trip; a
+// graph of 16 steps pays it once per 64 kernels.
+constexpr int kStepsPerGraph = 16;
+
+// A captured sequence of full steps, keyed on the buffers and dt it bakes in.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;
+  double dt = 0.0;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
 struct dgc_op_s {
   dgc_disc_s* disc = nullptr;
   int physics = 0;
@@ -64,11 +82,16 @@ struct dgc_op_s {
   int cache_policy = DGC_CACHE_AUTO;
   double* tclock = nullptr;
   int* flag = nullptr;  // non-finite flag of the time loops' state check
-  // cached graph of one full RK4 step
-  hipGraphExec_t graph = nullptr;
-  std::vector<const void*> graph_key;
-  double graph_dt = 0.0;
+  // cached graphs of one full RK4 step and of kStepsPerGraph steps in a row
+  StepGraph graph, chunk;
 };
+
+// Destroys the operator's cached step graphs (after a change of tiling,
+// precision or cache policy, which the captured launches bake in).
+static void drop_graphs(dgc_op_s* op) {
+  op->graph.reset();
+  op->chunk.reset();
+}
 
 extern "C" {
 
@@ -332,7 +355,7 @@ static int build_mfma_operands(dgc_op_s* op) {
 }
 
 static void free_op(dgc_op_s* op) {
-  if (op->graph) (void)hipGraphExecDestroy(op->graph);
+  drop_graphs(op);
   (void)hipFree(op->opF);
   (void)hipFree(op->opD);
   (void)hipFree(op->DrT);
@@ -627,22 +650,26 @@ static bool op_uses_time(const dgc_op_s* op) {
   return op->physics == dgc::kAdvection && op->bc == dgc::kBcSinSin && op->disc->num_boundary_nodes > 0;
 }
 
-// Build (or reuse) a graph holding one full step driven by the device clock.
-static int step_graph(dgc_op_s* op, double* const* u, double dt, double* const* wk) {
+// Build (or reuse) a graph holding `steps` full steps. An operator that reads
+// the time runs on the device clock: each step's kernels read the time the
+// previous step's tick left.
+static int step_graph(dgc_op_s* op, StepGraph& sg, int steps, double* const* u, double dt,
+                      double* const* wk) {
   std::vector<const void*> key;
   for (int f = 0; f < op->F; ++f) key.push_back(u[f]);
   for (int i = 0; i < 4 * op->F; ++i) key.push_back(wk[i]);
-  if (op->graph && key == op->graph_key && op->graph_dt == dt) return DGC_OK;
-  if (op->graph) {
-    (void)hipGraphExecDestroy(op->graph);
-    op->graph = nullptr;
-  }
+  if (sg.exec && key == sg.key && sg.dt == dt) return DGC_OK;
+  sg.reset();
   hipStream_t s = op->disc->ctx->stream;
   hipGraph_t g = nullptr;
   DGC_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(op, u, 0.0, dt, wk, true);
-  if (rc == DGC_OK && op_uses_time(op)) {
-    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, s, op->tclock, dt);
+  int rc = DGC_OK;
+  const bool clock = op_uses_time(op);
+  for (int i = 0; i < steps && rc == DGC_OK; ++i) {
+    rc = enqueue_step(op, u, 0.0, dt, wk, clock);
+    if (rc == DGC_OK && clock) {
+      hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, s, op->tclock, dt);
+    }
   }
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != DGC_OK) {
@@ -650,27 +677,50 @@ static int step_graph(dgc_op_s* op, double* const* u, double dt, double* const* 
     return rc;
   }
   if (e != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&op->graph, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    op->graph = nullptr;
+    sg.exec = nullptr;
     return dgc_set_error(DGC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
-  op->graph_key = key;
-  op->graph_dt = dt;
+  sg.key = key;
+  sg.dt = dt;
+  return DGC_OK;
+}
+
+// Queues nsteps full steps: nsteps / kStepsPerGraph launches of the chunk
+// graph, then the remainder from the one-step graph. The kernels and their
+// order are those of nsteps one-step launches.
+static hipError_t launch_steps(dgc_op_s* op, int nsteps) {
+  hipStream_t s = op->disc->ctx->stream;
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < nsteps / kStepsPerGraph && err == hipSuccess; ++i) err = hipGraphLaunch(op->chunk.exec, s);
+  for (int i = 0; i < nsteps % kStepsPerGraph && err == hipSuccess; ++i) err = hipGraphLaunch(op->graph.exec, s);
+  return err;
+}
+
+// Captures the graphs launch_steps(op, nsteps) replays.
+static int step_graphs_for(dgc_op_s* op, int nsteps, double* const* u, double dt, double* const* wk) {
+  if (nsteps >= kStepsPerGraph) {
+    int rc = step_graph(op, op->chunk, kStepsPerGraph, u, dt, wk);
+    if (rc != DGC_OK) return rc;
+  }
+  if (nsteps % kStepsPerGraph != 0) return step_graph(op, op->graph, 1, u, dt, wk);
   return DGC_OK;
 }
 
 static int run_full_steps(dgc_op_s* op, double* const* u, double t0, double dt, int nsteps,
                           double* const* wk) {
   if (nsteps <= 0) return DGC_OK;
-  int rc = step_graph(op, u, dt, wk);
+  int rc = step_graphs_for(op, nsteps, u, dt, wk);
   if (rc != DGC_OK) return rc;
-  hipStream_t s = op->disc->ctx->stream;
-  DGC_HIP_TRY(hipMemcpyAsync(op->tclock, &t0, sizeof(double), hipMemcpyHostToDevice, s));
-  // the host copy must stay alive until the copy has run
-  DGC_HIP_TRY(hipStreamSynchronize(s));
-  for (int i = 0; i < nsteps; ++i) DGC_HIP_TRY(hipGraphLaunch(op->graph, s));
+  if (op_uses_time(op)) {
+    hipStream_t s = op->disc->ctx->stream;
+    DGC_HIP_TRY(hipMemcpyAsync(op->tclock, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+    // the host copy must stay alive until the copy has run
+    DGC_HIP_TRY(hipStreamSynchronize(s));
+  }
+  DGC_HIP_TRY(launch_steps(op, nsteps));
   return DGC_OK;
 }
 
@@ -691,21 +741,18 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
   if (rc != DGC_OK) return rc;
   DGC_REQUIRE(nsteps > 0 && ms != nullptr, "dgc_rk4_steps_timed: need nsteps > 0 and ms");
   DGC_REQUIRE(std::isfinite(dt) && dt > 0.0, "dgc_rk4_steps_timed: dt must be positive");
-  rc = step_graph(op, inout, dt, wk);
+  rc = step_graphs_for(op, nsteps, inout, dt, wk);
   if (rc != DGC_OK) return rc;
   hipStream_t s = op->disc->ctx->stream;
   DGC_HIP_TRY(hipMemcpyAsync(op->tclock, &t0, sizeof(double), hipMemcpyHostToDevice, s));
   DGC_HIP_TRY(hipStreamSynchronize(s));
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  hipError_t err = hipEventRecord(e0, s);
-  for (int i = 0; i < nsteps && err == hipSuccess; ++i) err = hipGraphLaunch(op->graph, s);
-  if (err == hipSuccess) err = hipEventRecord(e1, s);
-  if (err == hipSuccess) err = hipEventSynchronize(e1);
-  if (err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  hipError_t err = hipEventRecord(ev.start, s);
+  if (err == hipSuccess) err = launch_steps(op, nsteps);
+  if (err == hipSuccess) err = hipEventRecord(ev.stop, s);
+  if (err == hipSuccess) err = hipEventSynchronize(ev.stop);
+  if (err == hipSuccess) err = hipEventElapsedTime(ms, ev.start, ev.stop);
   if (err != hipSuccess) return dgc_set_error(DGC_ERR_HIP, std::string("timed replay: ") + hipGetErrorString(err));
   return check_state_finite(op, inout, "dgc_rk4_steps_timed");
 }
@@ -715,10 +762,7 @@ int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
   DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == 8 || wt == 16 || wt == dgc::kMfmaTiling,
               "dgc_op_set_waves_per_tile: wt must be 1, 2, 4 or 0 (FP64 matrix-core variant)");
   op->wt = dgc::supported_waves_per_tile(op->disc->order, op->physics, wt);
-  if (op->graph) {
-    (void)hipGraphExecDestroy(op->graph);
-    op->graph = nullptr;
-  }
+  drop_graphs(op);
   return DGC_OK;
 }
 
@@ -730,10 +774,7 @@ int dgc_op_set_precision(dgc_op_t op, int precision) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_precision: null operator");
   DGC_REQUIRE(precision == 0 || precision == 1, "dgc_op_set_precision: precision must be 0 (FP64) or 1 (FP32 MFMA)");
   op->precision = precision;
-  if (op->graph) {
-    (void)hipGraphExecDestroy(op->graph);
-    op->graph = nullptr;
-  }
+  drop_graphs(op);
   return DGC_OK;
 }
 
@@ -744,10 +785,7 @@ int dgc_op_set_cache_policy(dgc_op_t op, int policy) {
   DGC_REQUIRE(policy == DGC_CACHE_AUTO || policy == DGC_CACHE_TEMPORAL || policy == DGC_CACHE_STREAMING,
               "dgc_op_set_cache_policy: policy must be DGC_CACHE_AUTO, DGC_CACHE_TEMPORAL or DGC_CACHE_STREAMING");
   op->cache_policy = policy;
-  if (op->graph) {
-    (void)hipGraphExecDestroy(op->graph);
-    op->graph = nullptr;
-  }
+  drop_graphs(op);
   return DGC_OK;
 }
 
@@ -1143,17 +1181,14 @@ int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms) {
       return dgc_set_error(DGC_ERR_HIP, std::string("dgc_ensemble_steps: instantiate: ") + hipGetErrorString(ce));
     }
   }
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  DGC_HIP_TRY(hipEventRecord(e0, st));
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  DGC_HIP_TRY(hipEventRecord(ev.start, st));
   for (int i = 0; i < nsteps; ++i) DGC_HIP_TRY(hipGraphLaunch(e->graph, st));
-  DGC_HIP_TRY(hipEventRecord(e1, st));
-  DGC_HIP_TRY(hipEventSynchronize(e1));
+  DGC_HIP_TRY(hipEventRecord(ev.stop, st));
+  DGC_HIP_TRY(hipEventSynchronize(ev.stop));
   float el = 0.f;
-  (void)hipEventElapsedTime(&el, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&el, ev.start, ev.stop);
   if (ms) *ms = el;
   return DGC_OK;
 }
@@ -1161,20 +1196,17 @@ int dgc_ensemble_steps(dgc_ensemble_t e, int nsteps, float* ms) {
 int dgc_ensemble_steps_sequential(dgc_ensemble_t e, int nsteps, float* ms) {
   DGC_REQUIRE(e && nsteps >= 0, "dgc_ensemble_steps_sequential: bad argument");
   hipStream_t st = e->ctx->stream;
-  hipEvent_t e0, e1;
-  DGC_HIP_TRY(hipEventCreate(&e0));
-  DGC_HIP_TRY(hipEventCreate(&e1));
-  DGC_HIP_TRY(hipEventRecord(e0, st));
+  dgc::EventPair ev;
+  DGC_HIP_TRY(ev.create());
+  DGC_HIP_TRY(hipEventRecord(ev.start, st));
   for (size_t i = 0; i < e->ops.size(); ++i) {
     int rc = run_full_steps(e->ops[i], e->state[i].data(), 0.0, e->dt[i], nsteps, e->work[i].data());
     if (rc != DGC_OK) return rc;
   }
-  DGC_HIP_TRY(hipEventRecord(e1, st));
-  DGC_HIP_TRY(hipEventSynchronize(e1));
+  DGC_HIP_TRY(hipEventRecord(ev.stop, st));
+  DGC_HIP_TRY(hipEventSynchronize(ev.stop));
   float el = 0.f;
-  (void)hipEventElapsedTime(&el, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&el, ev.start, ev.stop);
   if (ms) *ms = el;
   return DGC_OK;
 }

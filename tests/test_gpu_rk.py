@@ -99,9 +99,10 @@ def test_single_steps_equal_graph_replay(ctx, c1):
     b = upload_fields(ctx, u0, 3, K, Np)
     wa, wb = op.new_work(), op.new_work()
     dt = float(fix["dt"])
-    for i in range(5):
+    # 37 steps: two 16-step graph launches and five one-step launches
+    for i in range(37):
         op.rk4_step(a, i * dt, dt, wa)
-    op.rk4_steps(b, 0.0, dt, 5, wb)
+    op.rk4_steps(b, 0.0, dt, 37, wb)
     np.testing.assert_array_equal(download_fields(a, K, Np), download_fields(b, K, Np))
     # and one step against the reference's first step
     s = upload_fields(ctx, np.concatenate([np.zeros(2 * K * Np),
@@ -134,6 +135,16 @@ def test_advection_reference_problem(ctx, golden):
         op.rk4_step(s2, t, dt, work)
         t += dt
     np.testing.assert_array_equal(download_fields(s2, K, Np), download_fields(state, K, Np))
+    # 16-step graphs carry the device clock across their steps: 35 steps from
+    # the time loop equal 35 explicit-time steps
+    s3 = upload_fields(ctx, fix["ic"], 1, K, Np)
+    s4 = upload_fields(ctx, fix["ic"], 1, K, Np)
+    op.rk4_steps(s3, 0.25, dt, 35, work)
+    t = 0.25
+    for _ in range(35):
+        op.rk4_step(s4, t, dt, work)
+        t += dt
+    np.testing.assert_array_equal(download_fields(s3, K, Np), download_fields(s4, K, Np))
 
 
 @pytest.mark.parametrize("N", range(1, 8))
