@@ -128,8 +128,23 @@ constexpr int group_threads(int wt) { return kTile * (wt > kWaves ? wt : kWaves)
 
 // LDS (doubles) of one FP64 stage workgroup: node rows and face fluxes of its
 // tiles.
+// Maxwell stage kernels where the lift applies the face normal per face: the
+// tile keeps two flux values per face node (the upwind combination w / 2
+// shared by Hx and Hy, and the Ez flux) instead of three. Measured per order
+// and tiling (whole RK4 steps, K = 8.4M at N <= 3, K = 131072 above): N = 7
+// at 8 waves per tile 689 -> 513 us, N = 2 at 2 waves 4672 -> 4523 us; at
+// N = 3..6 the extra registers cost what the smaller LDS footprint gains.
+constexpr bool stage_factors_normal(int N, int physics, int wt) {
+  return physics == kMaxwellTM && ((N == 2 && wt <= 2) || N == 7);
+}
+// Node rows of the F fields plus the face fluxes.
+constexpr int stage_flux_fields(int N, int physics, int wt) {
+  return physics != kMaxwellTM ? 1 : (stage_factors_normal(N, physics, wt) ? 2 : 3);
+}
 constexpr int stage_lds_doubles(int N, int physics, int wt) {
-  return tiles_per_group(wt) * (physics == kMaxwellTM ? 3 : 1) * ((N + 1) * (N + 2) / 2 + 3 * (N + 1)) * kTile;
+  return tiles_per_group(wt) *
+         ((physics == kMaxwellTM ? 3 : 1) * (N + 1) * (N + 2) / 2 + stage_flux_fields(N, physics, wt) * 3 * (N + 1)) *
+         kTile;
 }
 
 // wt: waves per 64-cell tile (1, 2, 4, 8 or 16); a variant whose LDS would

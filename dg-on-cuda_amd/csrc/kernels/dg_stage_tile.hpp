@@ -65,10 +65,12 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
   constexpr bool kDelayOperands = stage_delays_operands(N, PHYS, WT);
   constexpr int kUnroll = stage_unroll(N, PHYS, WT);
+  constexpr bool kFactor = stage_factors_normal(N, PHYS, WT);
+  constexpr int FL = stage_flux_fields(N, PHYS, WT);
   using UsT = double[F][Np][kTile];
-  using FlT = double[F][NF][kTile];
+  using FlT = double[FL][NF][kTile];
   UsT* us = reinterpret_cast<UsT*>(lds);                        // us[TPW][F][Np][kTile]
-  FlT* fl = reinterpret_cast<FlT*>(lds + TPW * F * Np * kTile);  // fl[TPW][F][NF][kTile]
+  FlT* fl = reinterpret_cast<FlT*>(lds + TPW * F * Np * kTile);  // fl[TPW][FL][NF][kTile]
 
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
@@ -168,15 +170,20 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
       }
     }
     if (PHYS == kMaxwellTM) {
+      // Hx, Hy fluxes: (ny, -nx) * w / 2 with the face's normal
       const double dHx = a[0] - b[0];
       const double dHy = a[1 % F] - b[1 % F];
       const double sEz = a[2 % F] + b[2 % F];
       const double w = sEz + ny * dHx - nx * dHy;
-      fl[tw][0][m][lane] = 0.5 * ny * w * fs;
-      if (F == 3) {
-        fl[tw][1 % F][m][lane] = -0.5 * nx * w * fs;
-        fl[tw][2 % F][m][lane] =
-            0.5 * (ny * (a[0] + b[0]) - nx * (a[1 % F] + b[1 % F]) + a[2 % F] - b[2 % F]) * fs;
+      const double fez = 0.5 * (ny * (a[0] + b[0]) - nx * (a[1 % F] + b[1 % F]) + a[2 % F] - b[2 % F]) * fs;
+      if (kFactor) {
+        // the lift applies the normal per face: the tile keeps w / 2 once
+        fl[tw][0][m][lane] = 0.5 * w * fs;
+        fl[tw][1 % FL][m][lane] = fez;
+      } else {
+        fl[tw][0][m][lane] = 0.5 * ny * w * fs;
+        fl[tw][1 % FL][m][lane] = -0.5 * nx * w * fs;
+        fl[tw][2 % FL][m][lane] = fez;
       }
     } else {
       const double an = p.ax * nx + p.ay * ny;
@@ -257,17 +264,51 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   }
 
   // ---- 5. lift: res -= LIFT . (flux * face_J / J) ----------------------------
+  if constexpr (kFactor) {
+    // per face e: sw = LIFT_e . (w / 2), then Hx -= ny_e sw, Hy += nx_e sw
+    // (the face loop stays rolled where the code shape limits the unroll)
+    constexpr int kFaceUnroll = kUnroll < 64 ? 1 : 3;
+#pragma unroll kFaceUnroll
+    for (int e = 0; e < 3; ++e) {
+      double sw[RPW];
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) sw[i] = 0.0;
 #pragma unroll kUnroll
-  for (int m = 0; m < NF; ++m) {
-    double fm[F];
+      for (int d = 0; d < Nfp; ++d) {
+        const int m = e * Nfp + d;
+        const double fw = fl[tw][0][m][lane];
+        const double fe = fl[tw][1][m][lane];
+        const double* lt = p.LT + m * S + j0;
 #pragma unroll
-    for (int f = 0; f < F; ++f) fm[f] = fl[tw][f][m][lane];
-    const double* lt = p.LT + m * S + j0;
+        for (int i = 0; i < RPW; ++i) {
+          const double cl = lt[i];
+          sw[i] = fma(cl, fw, sw[i]);
+          res[2 % F][i] = fma(-cl, fe, res[2 % F][i]);
+        }
+      }
+      const double vx = e == 0 ? sx : (e == 1 ? rx + sx : rx);
+      const double vy = e == 0 ? sy : (e == 1 ? ry + sy : ry);
+      const double sg = (e == 1 ? 1.0 : -1.0) / (valid ? sqrt(vx * vx + vy * vy) : 1.0);
+      const double nx = sg * vx, ny = sg * vy;
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const double cl = lt[i];
+      for (int i = 0; i < RPW; ++i) {
+        res[0][i] = fma(-ny, sw[i], res[0][i]);
+        res[1 % F][i] = fma(nx, sw[i], res[1 % F][i]);
+      }
+    }
+  } else {
+#pragma unroll kUnroll
+    for (int m = 0; m < NF; ++m) {
+      double fm[F];
 #pragma unroll
-      for (int f = 0; f < F; ++f) res[f][i] = fma(-cl, fm[f], res[f][i]);
+      for (int f = 0; f < F; ++f) fm[f] = fl[tw][f][m][lane];
+      const double* lt = p.LT + m * S + j0;
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const double cl = lt[i];
+#pragma unroll
+        for (int f = 0; f < F; ++f) res[f][i] = fma(-cl, fm[f], res[f][i]);
+      }
     }
   }
 
