@@ -114,10 +114,6 @@ def mfma_stats(r, peak=MFMA_PEAK_F32_TFLOPS):
                 peak_tflops=peak, frac_issued=round(issued / dur / 1e12 / peak, 4))
 
 
-# HBM traffic per launch from FETCH_SIZE and WRITE_SIZE counter runs (scaled by the
-# calibration factors x2.000 and x1.000), keyed by (physics, N, K, waves per tile)
-# of the stage-2 kernel; profiles/dg_stage_kernel_advection_N7_K131072_pmc.md and
-# profiles/dg_stage_kernel_N3_K8388608_hbm.md hold the runs.
 # HBM traffic per stage-2 launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one run each, scaled x2.000 and x1.000 by the same counters on
 # dgc_axpy of known size): profiles/dg_stage_kernel_advection_N7_K131072_pmc.md
@@ -480,6 +476,7 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     instance after instance with the per-order kernels."""
     rng = np.random.default_rng(seed)
     ops, states, works, dts, dofs, orders, Ks = [], [], [], [], 0, [], []
+    step_bytes = 0  # algorithmic bytes of one RK4 step of every instance
     t0 = time.perf_counter()
     for i in range(n_inst):
         k_target = float(np.exp(rng.uniform(np.log(2e3), np.log(2e5))))
@@ -499,6 +496,7 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
         works.append(op.new_work())
         dts.append(0.5 * mesh.min_edge / (N + 1) ** 2)
         dofs += 3 * disc.K * disc.Np
+        step_bytes += sum(stage_bytes(disc.K, disc.Np, 3, st_) for st_ in (1, 2, 3, 4))
         orders.append(N)
         Ks.append(disc.K)
     t_setup = time.perf_counter() - t0
@@ -518,11 +516,14 @@ def ensemble_bench(dgc, ctx, n_inst=64, steps=500, seed=2):
     out = dict(instances=n_inst, total_dofs=dofs, steps=steps, K_min=min(Ks), K_max=max(Ks),
                orders={N: orders.count(N) for N in sorted(set(orders))},
                kernel_by_order={N: ("FP64 MFMA, 64 cells per workgroup" if w == 0 else
-                                    f"VALU, {64 * (4 // w)} cells per workgroup")
+                                    f"VALU, {64 * max(1, 4 // w)} cells per workgroup of {max(4, w)} waves")
                                 for N, w in sorted(wt_by_order.items())},
                work_items=info["num_items"], grid=info["grid"], lds_bytes=info["lds_bytes"],
                nontemporal_streams=ens.streaming,
-               ensemble=dict(value=v_e, ms_per_step=ms_e / steps),
+               step_bytes=step_bytes,
+               ensemble=dict(value=v_e, ms_per_step=ms_e / steps,
+                             GBs=round(step_bytes / (ms_e / steps * 1e-3) / 1e9, 1),
+                             frac_of_8TBs=round(step_bytes / (ms_e / steps * 1e-3) / 8e12, 4)),
                sequential=dict(value=v_s, ms_per_step=ms_s / steps),
                speedup=round(v_e / v_s, 3), finite=finite, setup_s=round(t_setup, 1),
                unit="DOF-stage updates/s (3 fields)")
