@@ -138,8 +138,9 @@ def roofline_for(r):
     phys = r.get("physics", "advection")
     traffic = MEASURED_TRAFFIC.get((phys, r["N"], r["K"], wt))
     hint = ", nontemporal streams" if r.get("streaming") else ""
+    tiling = "rows split over 4 workgroups of 4 waves" if wt == 32 else f"{wt} waves per tile"
     name = (f"dg_stage_mfma_kernel<double, N={r['N']}, {phys}, stage 2{hint}>" if wt == 0 else
-            f"dg_stage_kernel<N={r['N']}, {phys}, stage 2, {wt} waves per tile{hint}>")
+            f"dg_stage_kernel<N={r['N']}, {phys}, stage 2, {tiling}{hint}>")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_source=TRAFFIC_SOURCE if traffic else None, kernel=name,

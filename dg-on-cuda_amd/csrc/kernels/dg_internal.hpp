@@ -119,15 +119,22 @@ hipError_t launch_ensemble_stage(int order, int wt, bool streaming, int stage, c
 // Padded operator row count used by the stage kernels.
 inline int padded_rows(int Np) { return kWaves * ((Np + kWaves - 1) / kWaves); }
 
-// Tiles per workgroup and workgroup size of the FP64 stage kernels: with
-// wt = 1, 2 or 4 waves per tile a 256-thread workgroup holds 4/wt tiles; with
-// wt = 8 or 16 one tile has a 512- or 1024-thread workgroup to itself (fewer
-// output rows per wave: a shorter critical path on meshes of a few tiles).
+// Tilings of the FP64 stage kernels (the `wt` of the entry points). With
+// wt = 1, 2 or 4 waves per 64-cell tile a 256-thread workgroup holds 4/wt
+// tiles; with wt = 8 or 16 one tile has a 512- or 1024-thread workgroup to
+// itself (fewer output rows per wave: a shorter critical path on meshes of a
+// few tiles). kSplitTiling spreads one tile over kSplitGroups workgroups of 4
+// waves on as many CUs: each loads the tile and forms all its face fluxes,
+// and computes a quarter of the output rows, so on a mesh of a few tiles the
+// LDS reads and products of a tile are shared by four CUs instead of one.
+constexpr int kSplitTiling = 32;
+constexpr int kSplitGroups = 4;
+constexpr int tile_splits(int wt) { return wt == kSplitTiling ? kSplitGroups : 1; }
+// Waves of one workgroup that serve a tile.
+constexpr int tile_waves(int wt) { return wt == kSplitTiling ? kWaves : wt; }
 constexpr int tiles_per_group(int wt) { return wt >= kWaves ? 1 : kWaves / wt; }
-constexpr int group_threads(int wt) { return kTile * (wt > kWaves ? wt : kWaves); }
+constexpr int group_threads(int wt) { return kTile * (tile_waves(wt) > kWaves ? tile_waves(wt) : kWaves); }
 
-// LDS (doubles) of one FP64 stage workgroup: node rows and face fluxes of its
-// tiles.
 // Maxwell stage kernels where the lift applies the face normal per face: the
 // tile keeps two flux values per face node (the upwind combination w / 2
 // shared by Hx and Hy, and the Ez flux) instead of three. Measured per order
@@ -137,18 +144,21 @@ constexpr int group_threads(int wt) { return kTile * (wt > kWaves ? wt : kWaves)
 constexpr bool stage_factors_normal(int N, int physics, int wt) {
   return physics == kMaxwellTM && ((N == 2 && wt <= 2) || N == 7);
 }
-// Node rows of the F fields plus the face fluxes.
+// Flux values kept per face node.
 constexpr int stage_flux_fields(int N, int physics, int wt) {
   return physics != kMaxwellTM ? 1 : (stage_factors_normal(N, physics, wt) ? 2 : 3);
 }
+// LDS (doubles) of one FP64 stage workgroup: node rows of the F fields and
+// face fluxes of its tiles.
 constexpr int stage_lds_doubles(int N, int physics, int wt) {
   return tiles_per_group(wt) *
          ((physics == kMaxwellTM ? 3 : 1) * (N + 1) * (N + 2) / 2 + stage_flux_fields(N, physics, wt) * 3 * (N + 1)) *
          kTile;
 }
 
-// wt: waves per 64-cell tile (1, 2, 4, 8 or 16); a variant whose LDS would
-// not fit falls back to 4 (supported_waves_per_tile tells which one runs).
+// wt: waves per 64-cell tile (1, 2, 4, 8 or 16) or kSplitTiling; a variant
+// whose LDS would not fit falls back to 4 (supported_waves_per_tile tells
+// which one runs).
 hipError_t launch_stage(int order, int physics, int stage, int wt, const StageParams& p, hipStream_t s);
 int supported_waves_per_tile(int order, int physics, int wt);
 int default_waves_per_tile(int order, int physics, int K);

@@ -26,7 +26,8 @@ constexpr StageFn stage_fn(int stage) {
 
 // A variant is built only if its LDS (all tiles of the workgroup) stays
 // within 80 KB, so that at least two workgroups share a CU; larger requests
-// fall back to one tile per workgroup (WT = 4). WT = 8, 16 hold one tile.
+// fall back to one tile per workgroup (WT = 4). WT = 8, 16 and the split
+// tiling hold one tile.
 template <int N, int PHYS, int WT>
 constexpr bool fits() {
   return WT >= 4 || stage_lds_doubles(N, PHYS, WT) * 8 <= 80 * 1024;
@@ -42,6 +43,7 @@ StageFn pick_wt(int wt, int stage) {
   }
   if (wt == 8) return stage_fn<N, PHYS, 8, NT>(stage);
   if (wt == 16) return stage_fn<N, PHYS, 16, NT>(stage);
+  if (wt == kSplitTiling) return stage_fn<N, PHYS, kSplitTiling, NT>(stage);
   return stage_fn<N, PHYS, 4, NT>(stage);
 }
 

@@ -44,14 +44,15 @@ namespace dgc {
 // each launch is a chain of dependent latencies) is what counts: the Maxwell
 // cavity of 8 tiles at N = 4, 491 steps, 14.06 -> 13.02 ms.
 constexpr int stage_unroll(int N, int PHYS, int WT) {
-  if (PHYS == kMaxwellTM) return ((N == 2 && WT <= 2) || (WT == 16 && N <= 4)) ? 64 : 4;
+  if (PHYS == kMaxwellTM) return ((N == 2 && WT <= 2) || (WT >= 16 && N <= 4)) ? 64 : 4;
   return (WT == 8 && N >= 6) ? 2 : 64;
 }
 constexpr bool stage_delays_operands(int N, int PHYS, int WT) { return stage_unroll(N, PHYS, WT) < 64; }
 
 // One workgroup's share of a stage: the TPW = 4/WT tiles of tile group
 // `group` (cells group*64*TPW ...), with `lds` holding stage_lds_doubles()
-// doubles. Used by the per-order kernels below and by the ensemble kernel.
+// doubles; with the split tiling, part `group % RS` of the output rows of tile
+// `group / RS`. Used by the per-order kernels below and by the ensemble kernel.
 // NT: nontemporal hints on the accumulator streams (ld_once / st_once).
 template <int N, int PHYS, int STAGE, int WT, bool NT>
 __device__ __forceinline__ void stage_tile(const StageParams& p, int group, double* lds) {
@@ -59,9 +60,11 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   constexpr int Np = T::Np, Nfp = T::Nfp, NF = 3 * Nfp;
   constexpr int F = PHYS == kMaxwellTM ? 3 : 1;
   constexpr int TPW = tiles_per_group(WT);  // tiles per workgroup
-  constexpr int RPW = (Np + WT - 1) / WT;  // output rows per wave (contiguous block)
-  constexpr int LPW = (Np + WT - 1) / WT;  // input rows loaded per wave (strided)
-  constexpr int FPW = (NF + WT - 1) / WT;  // face slots per wave (strided)
+  constexpr int WL = tile_waves(WT);        // waves of this workgroup per tile
+  constexpr int RS = tile_splits(WT);       // workgroups sharing a tile's output rows
+  constexpr int RPW = (Np + WL * RS - 1) / (WL * RS);  // output rows per wave (contiguous block)
+  constexpr int LPW = (Np + WL - 1) / WL;  // input rows loaded per wave (strided)
+  constexpr int FPW = (NF + WL - 1) / WL;  // face slots per wave (strided)
   constexpr int S = kWaves * ((Np + kWaves - 1) / kWaves);  // operator row stride
   constexpr bool kDelayOperands = stage_delays_operands(N, PHYS, WT);
   constexpr int kUnroll = stage_unroll(N, PHYS, WT);
@@ -75,9 +78,10 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   const int K = p.K;
   const int lane = threadIdx.x & (kTile - 1);
   const int wave = uniform(threadIdx.x >> 6);
-  const int tw = wave / WT;
-  const int wt = wave - tw * WT;
-  const int c0 = (group * TPW + tw) * kTile;
+  const int tw = wave / WL;
+  const int wt = wave - tw * WL;
+  const int part = RS > 1 ? group % RS : 0;  // output-row part of the split tiling
+  const int c0 = ((RS > 1 ? group / RS : group) * TPW + tw) * kTile;
   const int c = c0 + lane;
   const bool valid = c < K;
 
@@ -88,13 +92,13 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   int code[FPW];
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
+    const int m = wt + WL * i;
     code[i] = (valid && m < NF) ? p.nbr[(size_t)(m / Nfp) * K + c] : kBoundaryCode;
   }
   double tl[F][LPW];
 #pragma unroll
   for (int i = 0; i < LPW; ++i) {
-    const int j = wt + WT * i;
+    const int j = wt + WL * i;
 #pragma unroll
     for (int f = 0; f < F; ++f) tl[f][i] = (valid && j < Np) ? p.in[f][(size_t)j * K + c] : 0.0;
   }
@@ -108,7 +112,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   double gv[F][FPW];  // neighbour traces outside the tile
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
+    const int m = wt + WL * i;
     const int e = m / Nfp, d = m - e * Nfp;
     const int nf = code[i] & 3, idx = code[i] >> 2;
     const bool far = nf != kBoundaryCode && (unsigned)(idx - c0) >= (unsigned)kTile;
@@ -119,7 +123,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   }
 #pragma unroll
   for (int i = 0; i < LPW; ++i) {
-    const int j = wt + WT * i;
+    const int j = wt + WL * i;
     if (j < Np) {
 #pragma unroll
       for (int f = 0; f < F; ++f) us[tw][f][j][lane] = tl[f][i];
@@ -130,7 +134,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
   // ---- 2. numerical fluxes at the face nodes, scaled by face_J / J ---------
 #pragma unroll
   for (int i = 0; i < FPW; ++i) {
-    const int m = wt + WT * i;
+    const int m = wt + WL * i;
     if (m >= NF) break;
     const int e = m / Nfp, d = m - e * Nfp;
     const int own = T::fnode(e, d);
@@ -195,7 +199,7 @@ __device__ __forceinline__ void stage_tile(const StageParams& p, int group, doub
 
   // ---- 3. stage operands of this wave's rows (in flight during the FMAs, or
   //         loaded after the products where the code shape says so) ---------
-  const int j0 = wt * RPW;
+  const int j0 = (part * WL + wt) * RPW;
   if (j0 >= Np) return;  // a wave without output rows (no barrier follows)
   double o1[F][RPW], o2[F][RPW], o3[F][RPW];
   auto load_operands = [&]() {

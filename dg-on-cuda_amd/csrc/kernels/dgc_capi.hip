@@ -759,8 +759,10 @@ int dgc_rk4_steps_timed(dgc_op_t op, double* const* inout, size_t size, double t
 
 int dgc_op_set_waves_per_tile(dgc_op_t op, int wt) {
   DGC_REQUIRE(op != nullptr, "dgc_op_set_waves_per_tile: null operator");
-  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == 8 || wt == 16 || wt == dgc::kMfmaTiling,
-              "dgc_op_set_waves_per_tile: wt must be 1, 2, 4 or 0 (FP64 matrix-core variant)");
+  DGC_REQUIRE(wt == 1 || wt == 2 || wt == 4 || wt == 8 || wt == 16 || wt == dgc::kSplitTiling ||
+                  wt == dgc::kMfmaTiling,
+              "dgc_op_set_waves_per_tile: wt must be 1, 2, 4, 8, 16, 32 (rows of a tile split over 4 "
+              "workgroups) or 0 (FP64 matrix-core variant)");
   op->wt = dgc::supported_waves_per_tile(op->disc->order, op->physics, wt);
   drop_graphs(op);
   return DGC_OK;
@@ -847,10 +849,10 @@ int dgc_op_autotune(dgc_op_t op, double* const* inout, size_t size, double* cons
   dgc::StageParams p[5];
   timing_params(op, inout, wk, p);
   hipStream_t st = op->disc->ctx->stream;
-  // 8 and 16 waves per tile (one tile per 512- or 1024-thread workgroup):
-  // timed like the others, not reported in ms_out
-  const int ncand = 6;
-  const int cand[6] = {1, 2, 4, dgc::kMfmaTiling, 8, 16};
+  // 8 and 16 waves per tile (one tile per 512- or 1024-thread workgroup) and
+  // the split tiling: timed like the others, not reported in ms_out
+  const int ncand = 7;
+  const int cand[7] = {1, 2, 4, dgc::kMfmaTiling, 8, 16, dgc::kSplitTiling};
   float best = 0.f;
   int best_wt = op->wt;
   hipEvent_t e0, e1, e2;

@@ -510,7 +510,7 @@ class _Operator:
         check(lib().dgc_op_autotune(self._h, self._fp(state), self.disc.size, _ptrs([w.ptr for w in work]),
                                     reps, ms))
         out = dict(choice=self.waves_per_tile, ms={wt: m for wt, m in zip((1, 2, 4, 0), ms) if m >= 0})
-        if out["choice"] not in out["ms"]:  # 8 or 16 waves per tile
+        if out["choice"] not in out["ms"]:  # 8 or 16 waves per tile, or the split tiling (32)
             out["ms"][out["choice"]] = self.stage_time(state, work, 2, reps)
         return out
 

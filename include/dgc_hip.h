@@ -236,8 +236,10 @@ int dgc_op_num_fields(dgc_op_t op);
 
 /* Tiling of the stage kernels: wt waves per 64-cell tile (1, 2 or 4: 4/wt
  * tiles per 256-thread workgroup; 8 or 16: one tile per 512- or 1024-thread
- * workgroup, for meshes of a few tiles; the reference's launch-shape
- * arguments, main.cpp argv[2], play this role), or wt = 0 for the FP64
+ * workgroup, for meshes of a few tiles; 32: the output rows of each tile split
+ * over 4 workgroups of 4 waves, each of which loads the tile and forms its
+ * fluxes; the reference's launch-shape arguments, main.cpp argv[2], play this
+ * role), or wt = 0 for the FP64
  * matrix-core variant (Dr u, Ds u and LIFT flux as 16x16x4 f64 MFMA
  * products, one 64-cell tile per workgroup; FP64 throughout). A tiling whose
  * LDS would not fit falls back to 4. dgc_op_autotune times an RK4 step (the 4

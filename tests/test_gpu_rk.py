@@ -73,15 +73,16 @@ def test_maxwell_cavity_t1_matches_reference(ctx, c1):
     assert np.mean((got - exact) ** 2) < 1e-8
 
 
-@pytest.mark.parametrize("wt", [4, 8, 16])
+@pytest.mark.parametrize("wt", [4, 8, 16, 32])
 def test_maxwell_cavity_t1_every_small_mesh_tiling(ctx, golden, wt):
-    """The cavity run at 4, 8 and 16 waves per tile (the default on this
-    8-tile mesh is 16): the golden t = 1 state to the FP64 bar."""
+    """The cavity run at 4, 8 and 16 waves per tile and with each tile's rows
+    split over 4 workgroups (32, the default on this 8-tile mesh at N = 4):
+    the golden t = 1 state to the FP64 bar."""
     fix = golden("maxwell_c1.npz")
     mesh = dgc_amd.Mesh.from_arrays(fix["vx"], fix["vy"], fix["etov"])
     disc = dgc_amd.Discretization(ctx, mesh, int(fix["order"]))
     op = dgc_amd.MaxwellTM(disc)
-    assert op.waves_per_tile == 16
+    assert op.waves_per_tile == 32
     op.waves_per_tile = wt
     K, Np = disc.K, disc.Np
     u0 = np.concatenate([np.zeros(2 * K * Np), np.sin(np.pi * fix["x"]) * np.sin(np.pi * fix["y"])])
@@ -362,16 +363,17 @@ def test_bad_arguments_are_errors(ctx, c1):
         op.rk4_run(a, 0.0, 1.0, -1.0, op.new_work())
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 5])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7])
 def test_every_tiling_gives_identical_results(ctx, N):
-    """The stage kernels' tilings (1, 2, 4, 8 or 16 waves per tile, and 0: the
-    FP64 matrix-core variant) agree to rounding (the summation order and
-    multiply-add contraction differ per variant)."""
+    """The stage kernels' tilings (1, 2, 4, 8 or 16 waves per tile, 32: the
+    rows of a tile split over 4 workgroups, and 0: the FP64 matrix-core
+    variant) agree to rounding (the summation order and multiply-add
+    contraction differ per variant) on a mesh whose last tile is partial."""
     mesh = dgc_amd.Mesh.structured(20, 13, 0, 1, 0, 1, jitter=0.05, seed=4, periodic=True)
     disc = dgc_amd.Discretization(ctx, mesh, N)
     results = []
     for phys in ("adv", "mx"):
-        for wt in (1, 2, 4, 0, 8, 16):
+        for wt in (1, 2, 4, 0, 8, 16, 32):
             op = (dgc_amd.Advection(disc, 1.0, 0.5, dgc_amd.Advection.BC_ZERO) if phys == "adv"
                   else dgc_amd.MaxwellTM(disc))
             op.waves_per_tile = wt
