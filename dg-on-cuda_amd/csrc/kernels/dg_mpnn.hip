@@ -1,9 +1,10 @@
 // Learned preconditioner kernels (network defined in dg_mpnn.hpp).
 //
 // Forward layers run on the matrix cores: a wave owns 32 cells and computes
-// their 64 hidden units as two 32x32 tiles of v_mfma_f32_32x32x8bf16_1k over
+// their 64 hidden units as two 32x32 tiles of v_mfma_f32_32x32x16_bf16 over
 // the concatenated [own | neighbour-sum] inputs; the layer's weights are
-// pre-packed in the B-operand order and staged in LDS once per workgroup.
+// pre-packed in the B-operand order of k-steps of 8 and staged in LDS once
+// per workgroup.
 // Backward and weight-gradient kernels read the same bf16 data and
 // accumulate in fp32; weight gradients are reduced per workgroup into a
 // partial array and summed in a fixed order (deterministic training).
@@ -90,7 +91,15 @@ __global__ __launch_bounds__(256) void features_kernel(MpnnNet net, const double
 template <int D>
 __global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, const uint16_t* __restrict__ hin,
                                                         uint16_t* __restrict__ hout, const double* cgscal) {
-  constexpr int KS = D / 4;  // k-steps of 8 over [own (D) | neighbour sum (D)]
+  // k runs over [own (D) | neighbour sum (D)]. The weights are packed for
+  // k-steps of 8 (x8 step q, half h: wl[(2 q + h) 64 + lane]); the products
+  // take k-steps of 16 on gfx950's 32x32x16 instruction, whose B operand for
+  // lane (j, kq) is the two x8 entries of step 2 s + kq at lanes j and j + 32,
+  // and a last k-step of 8 where D is not a multiple of 16 (D = 24).
+  constexpr int KS = D / 4;   // x8 k-steps over both halves
+  constexpr int S16 = D / 16;  // x16 k-steps per half
+  constexpr bool kTail = (D % 16) != 0;
+  static_assert(D % 8 == 0, "width must be a multiple of 8");
   __shared__ uint64_t wl[KS * 2 * 64];
   if (cgscal && cgscal[kDone] != 0.0) return;
   const uint64_t* wp = net.wpk[layer];
@@ -109,42 +118,63 @@ __global__ __launch_bounds__(256) void dense_fwd_kernel(MpnnNet net, int layer, 
     nb[e] = (code & 3) != kBoundaryCode ? (code >> 2) : -1;
   }
   // every global read of the wave first (own rows, then the neighbour rows
-  // and their bf16-rounded sums), the products after: one round trip
-  uint64_t own[D / 8], nsum[D / 8];
+  // and their bf16-rounded sums), the products after: one round trip.
+  // Slot q < 2 S16 holds the 4 values at k = 16 (q / 2) + 8 kq + 4 (q % 2)
+  // (the x16 operand is slots 2 s, 2 s + 1); slot 2 S16 the tail's 4 values
+  // at k = 16 S16 + 4 kq.
+  constexpr int NS = 2 * S16 + (kTail ? 1 : 0);
+  uint64_t own[NS], nsum[NS];
+  auto koff = [&](int q) { return q < 2 * S16 ? 16 * (q / 2) + 8 * kq + 4 * (q % 2) : 16 * S16 + 4 * kq; };
 #pragma unroll
-  for (int ks = 0; ks < D / 8; ++ks)
-    own[ks] = valid ? *reinterpret_cast<const uint64_t*>(hin + (size_t)c * D + ks * 8 + 4 * kq) : 0;
+  for (int q = 0; q < NS; ++q)
+    own[q] = valid ? *reinterpret_cast<const uint64_t*>(hin + (size_t)c * D + koff(q)) : 0;
 #pragma unroll
-  for (int ks = 0; ks < D / 8; ++ks) {
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < NS; ++q) {
+    float sm[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
       if (nb[e] >= 0) {
         float t[4];
-        load_bf16x4(hin + (size_t)nb[e] * D + ks * 8 + 4 * kq, t);
-        for (int v = 0; v < 4; ++v) s[v] += t[v];
+        load_bf16x4(hin + (size_t)nb[e] * D + koff(q), t);
+        for (int v = 0; v < 4; ++v) sm[v] += t[v];
       }
     }
     uint64_t u = 0;
-    for (int v = 0; v < 4; ++v) u |= uint64_t(f2bf(s[v])) << (16 * v);
-    nsum[ks] = u;
+    for (int v = 0; v < 4; ++v) u |= uint64_t(f2bf(sm[v])) << (16 * v);
+    nsum[q] = u;
   }
   floatx16 acc0 = {}, acc1 = {};
+  const int j = lane & 31;
+  // B operand of x16 step s of half `base` (0: own, KS / 2: neighbour sum)
+  auto bop = [&](int base, int s16, int h) {
+    const int q = (base + 2 * s16 + kq) * 2 + h;
+    return as_bf16x8(wl[q * 64 + j], wl[q * 64 + 32 + j]);
+  };
 #pragma unroll
-  for (int ks = 0; ks < D / 8; ++ks) {
-    const bf16x4 a = as_bf16x4(own[ks]);
-    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 0) * 64 + lane]), acc0);
-    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(ks * 2 + 1) * 64 + lane]), acc1);
+  for (int s16 = 0; s16 < S16; ++s16) {
+    const bf16x8 a = as_bf16x8(own[2 * s16], own[2 * s16 + 1]);
+    acc0 = mfma32x32x16bf16(a, bop(0, s16, 0), acc0);
+    acc1 = mfma32x32x16bf16(a, bop(0, s16, 1), acc1);
+  }
+  if constexpr (kTail) {
+    const bf16x4 a = as_bf16x4(own[2 * S16]);
+    const int q = KS / 2 - 1;  // last x8 step of the own half
+    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(q * 2 + 0) * 64 + lane]), acc0);
+    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(q * 2 + 1) * 64 + lane]), acc1);
   }
 #pragma unroll
-  for (int ks = 0; ks < D / 8; ++ks) {
-    const bf16x4 a = as_bf16x4(nsum[ks]);
-    const int kk = D / 8 + ks;
-    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 0) * 64 + lane]), acc0);
-    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(kk * 2 + 1) * 64 + lane]), acc1);
+  for (int s16 = 0; s16 < S16; ++s16) {
+    const bf16x8 a = as_bf16x8(nsum[2 * s16], nsum[2 * s16 + 1]);
+    acc0 = mfma32x32x16bf16(a, bop(KS / 2, s16, 0), acc0);
+    acc1 = mfma32x32x16bf16(a, bop(KS / 2, s16, 1), acc1);
+  }
+  if constexpr (kTail) {
+    const bf16x4 a = as_bf16x4(nsum[2 * S16]);
+    const int q = KS - 1;  // last x8 step of the neighbour half
+    acc0 = mfma32x32x8bf16(a, as_bf16x4(wl[(q * 2 + 0) * 64 + lane]), acc0);
+    acc1 = mfma32x32x8bf16(a, as_bf16x4(wl[(q * 2 + 1) * 64 + lane]), acc1);
   }
   ConstF32* bias = weight_f32(net.wf + net.L.ob[layer]);
-  const int j = lane & 31;
   const float b0 = bias[j], b1 = bias[32 + j];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {

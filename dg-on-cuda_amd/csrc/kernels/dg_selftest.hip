@@ -22,6 +22,19 @@ __global__ void mfma_bf16_probe(const float* A, const float* B, float* C) {
   for (int r = 0; r < 16; ++r) C[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = d[r];
 }
 
+__global__ void mfma_bf16x16_probe(const float* A, const float* B, float* C) {
+  // A: 32x16 row-major, B: 16x32 row-major, C: 32x32 row-major
+  const int l = threadIdx.x;
+  uint64_t a[2] = {0, 0}, b[2] = {0, 0};
+  for (int v = 0; v < 8; ++v) {
+    a[v / 4] |= uint64_t(dgc::f2bf(A[(l % 32) * 16 + 8 * (l / 32) + v])) << (16 * (v % 4));
+    b[v / 4] |= uint64_t(dgc::f2bf(B[(8 * (l / 32) + v) * 32 + l % 32])) << (16 * (v % 4));
+  }
+  dgc::floatx16 d = {};
+  d = dgc::mfma32x32x16bf16(dgc::as_bf16x8(a[0], a[1]), dgc::as_bf16x8(b[0], b[1]), d);
+  for (int r = 0; r < 16; ++r) C[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = d[r];
+}
+
 __global__ void mfma_f32_probe(const float* A, const float* B, float* C) {
   // A: 16x4, B: 4x16, C: 16x16, row-major
   const int l = threadIdx.x;
@@ -43,9 +56,11 @@ __global__ void mfma_f64_probe(const float* A, const float* B, float* C) {
 }  // namespace
 
 extern "C" int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C) {
-  if (!ctx || !A || !B || !C || kind < 0 || kind > 2)
+  if (!ctx || !A || !B || !C || kind < 0 || kind > 3)
     return dgc_set_error(DGC_ERR_INVALID, "dgc_selftest_mfma: bad argument");
-  const int na = kind == 0 ? 32 * 8 : 16 * 4, nb = kind == 0 ? 8 * 32 : 4 * 16, nc = kind == 0 ? 32 * 32 : 16 * 16;
+  const int na = kind == 0 ? 32 * 8 : (kind == 3 ? 32 * 16 : 16 * 4);
+  const int nb = kind == 0 ? 8 * 32 : (kind == 3 ? 16 * 32 : 4 * 16);
+  const int nc = (kind == 0 || kind == 3) ? 32 * 32 : 16 * 16;
   float *dA = nullptr, *dB = nullptr, *dC = nullptr;
   hipError_t e = hipMalloc(&dA, na * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&dB, nb * sizeof(float));
@@ -58,6 +73,8 @@ extern "C" int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const 
       hipLaunchKernelGGL(mfma_bf16_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
     else if (kind == 1)
       hipLaunchKernelGGL(mfma_f32_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    else if (kind == 3)
+      hipLaunchKernelGGL(mfma_bf16x16_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
     else
       hipLaunchKernelGGL(mfma_f64_probe, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
     e = hipGetLastError();

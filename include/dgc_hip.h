@@ -495,8 +495,10 @@ int dgc_ipdg_fcg(dgc_ipdg_t s, dgc_mpnn_t m, const double* b, double* x, double 
 /* Device self-test of the matrix-core operand layouts the kernels rely on:
  * kind 0: C[32x32] = A[32x8] B[8x32] with bf16 inputs (rounded on the device),
  * kind 1: C[16x16] = A[16x4] B[4x16] in fp32, kind 2: the same in fp64
- * (inputs widened, result rounded to fp32). Row-major host arrays; the
- * result is scattered with the documented lane layout. No reference counterpart. */
+ * (inputs widened, result rounded to fp32; C holds the raw result registers),
+ * kind 3: C[32x32] = A[32x16] B[16x32] with bf16 inputs (gfx950's 32x32x16
+ * instruction). Row-major host arrays; the result is scattered with the
+ * documented lane layout. No reference counterpart. */
 int dgc_selftest_mfma(dgc_ctx_t ctx, int kind, const float* A, const float* B, float* C);
 
 /* Same as dgc_rk4_steps, bracketed by HIP events on the context stream;

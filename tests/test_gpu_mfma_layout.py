@@ -1,5 +1,6 @@
-"""Matrix-core operand layouts used by the MFMA kernels (bf16 32x32x8, fp32
-16x16x4 and fp64 16x16x4) checked on the device against a host product."""
+"""Matrix-core operand layouts used by the MFMA kernels (bf16 32x32x8 and
+32x32x16, fp32 16x16x4 and fp64 16x16x4) checked on the device against a host
+product."""
 import ctypes as C
 
 import numpy as np
@@ -16,18 +17,18 @@ def _bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_mfma_layout(ctx, kind):
     L = dgc_amd.lib()
     L.dgc_selftest_mfma.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     rng = np.random.default_rng(kind)
-    m, k, n = (32, 8, 32) if kind == 0 else (16, 4, 16)
+    m, k, n = {0: (32, 8, 32), 3: (32, 16, 32)}.get(kind, (16, 4, 16))
     A = rng.standard_normal((m, k)).astype(np.float32)
     B = rng.standard_normal((k, n)).astype(np.float32)
     Cm = np.zeros((m, n), np.float32)
     rc = L.dgc_selftest_mfma(ctx.handle, kind, A.ctypes.data, B.ctypes.data, Cm.ctypes.data)
     assert rc == 0, L.dgc_last_error()
-    if kind == 0:
+    if kind in (0, 3):
         A, B = _bf16_round(A), _bf16_round(B)
     want = A.astype(np.float64) @ B.astype(np.float64)
     if kind == 2:
